@@ -1,0 +1,156 @@
+/*
+ * gsqg.h -- C ABI of the MI355X-native pseudo-spectral gSQG time step.
+ *
+ * Every entry point returns an int status (GSQG_OK on success) and never
+ * throws.  Pointers named *_dev are device pointers (hipMalloc'd memory or
+ * torch CUDA tensors); full spectra are interleaved complex128, C order,
+ * shape (n, n), axis 0 = kx, axis 1 = ky -- the layout of the reference's
+ * `state["theta_hat"]`.  Physical fields are float64 (n, n), axis 0 = x.
+ * `stream` is a hipStream_t (NULL = the legacy default stream).  All calls
+ * are asynchronous on `stream` unless stated otherwise.
+ *
+ * A plan is not thread-safe; distinct plans are independent.  A plan holds
+ * one device-resident solver state (theta and its passive scalars).
+ *
+ * Citations refer to pygSQuiG (dfielding14/pygSQuiG) paths.
+ */
+#ifndef GSQG_H
+#define GSQG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GSQG_OK 0
+#define GSQG_ERR_ARG 1         /* invalid argument (message via gsqg_last_error) */
+#define GSQG_ERR_HIP 2         /* HIP runtime error */
+#define GSQG_ERR_NONFINITE 3   /* a non-finite state was produced */
+#define GSQG_ERR_UNSUPPORTED 4 /* valid request outside what this build supports */
+
+#define GSQG_SCHEME_EXPLICIT 0 /* reference RK4, hyperviscosity inside each stage */
+#define GSQG_SCHEME_IF 1       /* integrating-factor RK4 (new; exact linear decay) */
+
+#define GSQG_COMBINE_STEP 0      /* y + dt*(k1+2k2+2k3+k4)/6  (core/time_integrator.py:36) */
+#define GSQG_COMBINE_MULTISTEP 1 /* y + (dt/6)*(k1+2k2+2k3+k4) (core/solver_optimized.py:62) */
+
+#define GSQG_KEY_EXPLICIT 0 /* one caller key for the step(s) */
+#define GSQG_KEY_DERIVED 1  /* per-step key = step_key(seed, step index) */
+
+typedef struct gsqg_plan gsqg_plan;
+
+typedef struct gsqg_plan_desc {
+  int n;           /* points per side; power of two in [16, 8192] */
+  double L;        /* box length (core/grid.py:72) */
+  double alpha;    /* gSQG exponent, [-2, 2] (core/solver.py:91-92) */
+  double nu_p;     /* solver hyperviscosity (core/solver.py:75) */
+  int p;           /* 2, 4 or 8 (core/solver.py:93-94) */
+  int scheme;      /* GSQG_SCHEME_* */
+  int combine;     /* GSQG_COMBINE_* */
+  int hv_dc_zero;  /* 1: hyperviscosity zeroed at k=0 (core/operators.py:205) */
+  /* CombinedDamping(mu, kf, nu_p, p) (forcing/damping.py:49-92) */
+  int damping;
+  double damp_mu;
+  double damp_nu_p;
+  int damp_p;
+  /* RingForcing(kf, dk, epsilon, amplitude) (forcing/ring_forcing.py:31-62); tau_f = 0 */
+  int forcing;
+  double epsilon;
+  double amplitude;
+  /* passive scalars (scalars/passive_scalar.py:106-132); kappa[n_scalars] */
+  int n_scalars;
+  const double* kappa;
+  /* Host arrays computed with the reference's operation order:
+   *   k1d[n]: fftfreq(n, L/n) * 2 * pi  (core/grid.py:94)
+   *   mode_flags[(n/2) * n]: per half-plane mode, row = ky index, column = kx
+   *     index; bit0 dealias (grid.py:102-105), bit1 |k| < kf/2
+   *     (damping.py:34), bit2 forcing ring (ring_forcing.py:74-75).  Row 0
+   *     holds ky = 0 in the low nibble and ky = n/2 in the high nibble.
+   *   ring_modes[2 * n_ring]: (kx index, ky index <= n/2) of every mode that
+   *     carries the anti-Hermitian part of theta_hat (the forcing ring). */
+  const double* k1d;
+  const uint8_t* mode_flags;
+  const int32_t* ring_modes;
+  int n_ring;
+} gsqg_plan_desc;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int gsqg_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* gsqg_last_error(void);
+
+/* Allocate a plan and its device work space on the current HIP device.
+ * Replaces the set-up done by gSQGSolver.__init__ (core/solver.py:75-94),
+ * RingForcing/CombinedDamping construction and make_grid (core/grid.py:72-107). */
+int gsqg_plan_create(const gsqg_plan_desc* desc, gsqg_plan** out);
+int gsqg_plan_destroy(gsqg_plan* plan);
+
+/* Bytes of device memory held by the plan. */
+int gsqg_plan_device_bytes(const gsqg_plan* plan, size_t* bytes);
+
+/* Load theta_hat (full (n, n) complex128) into the plan's state.  The
+ * Hermitian part drives the nonlinear dynamics; the anti-Hermitian part is
+ * kept on the ring modes.  *offring_sq (may be NULL) receives the squared
+ * norm of the anti-Hermitian part off those modes, which the state cannot
+ * carry (synchronises `stream` when non-NULL).
+ * Input side of gSQGSolver.step (core/solver.py:204). */
+int gsqg_load_theta(gsqg_plan* plan, const void* theta_hat_dev, double* offring_sq, void* stream);
+
+/* Write the plan's theta_hat as a full (n, n) complex128 spectrum.
+ * Output side of gSQGSolver.step (core/solver.py:214). */
+int gsqg_store_theta(gsqg_plan* plan, void* theta_hat_dev, void* stream);
+
+/* Same for passive scalar `idx` (scalars/state.py:59-118 MultiScalarState.scalars). */
+int gsqg_load_scalar(gsqg_plan* plan, int idx, const void* scalar_hat_dev, double* offring_sq,
+                     void* stream);
+int gsqg_store_scalar(gsqg_plan* plan, int idx, void* scalar_hat_dev, void* stream);
+
+/* Advance the plan's state by n_steps RK4 steps of size dt: theta (with the
+ * plan's forcing and damping), then every passive scalar with the velocity
+ * of the new theta frozen over its four stages.
+ * Replaces gSQGSolver.step (core/solver.py:182-214), gSQGSolverOptimized
+ * .multistep (core/solver_optimized.py:168-199) and gSQGSolverWithScalars
+ * .step (core/solver_with_scalars.py:131-178).
+ * key_mode GSQG_KEY_EXPLICIT: `key` drives the forcing phases of every step
+ * (the reference passes one key per step); GSQG_KEY_DERIVED: step s uses
+ * step_key(key, step0 + s).  use_graph != 0 captures one step in a HIP graph
+ * and replays it (n_steps > 1). */
+int gsqg_advance(gsqg_plan* plan, double dt, int n_steps, int key_mode, uint64_t key,
+                 int64_t step0, int use_graph, void* stream);
+
+/* One RK4 step of every passive scalar of the plan with the given frozen
+ * physical velocity (u_dev, v_dev: (n, n) float64).
+ * Replaces MultiSpeciesEvolver.step / PassiveScalarEvolver.step
+ * (scalars/passive_scalar.py:212-258, 341-376) without source terms. */
+int gsqg_advance_scalars(gsqg_plan* plan, const double* u_dev, const double* v_dev, double dt,
+                         void* stream);
+
+/* rhs_dev = compute_rhs(theta_hat) with the plan's forcing/damping
+ * (core/solver.py:150-180); key as in gsqg_advance with one step. */
+int gsqg_rhs(gsqg_plan* plan, const void* theta_hat_dev, void* rhs_dev, uint64_t key,
+             void* stream);
+
+/* (u, v) physical from theta_hat (core/operators.py:158-181). */
+int gsqg_velocity(gsqg_plan* plan, const void* theta_hat_dev, double* u_dev, double* v_dev,
+                  void* stream);
+
+/* fft2 of a real field to a full complex spectrum (core/grid.py:110-124) and
+ * ifft2(...).real (core/grid.py:127-138). */
+int gsqg_fft2(gsqg_plan* plan, const double* field_dev, void* spectrum_dev, void* stream);
+int gsqg_ifft2(gsqg_plan* plan, const void* spectrum_dev, double* field_dev, void* stream);
+
+/* Non-finite flag raised by the last steps (synchronises stream). */
+int gsqg_nonfinite(gsqg_plan* plan, int* flag, int clear, void* stream);
+
+/* Ring-forcing <theta f> rate and scale of stage 1..4 of the last step
+ * (synchronises stream); rate4/scale4 are 4-element arrays. */
+int gsqg_forcing_stats(gsqg_plan* plan, double* rate4, double* scale4, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GSQG_H */

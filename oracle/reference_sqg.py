@@ -240,7 +240,10 @@ def if_rk4_step(state, nonlin_fn: Callable, lin: np.ndarray, dt: float):
     k3 = nonlin_fn(y3)
     y4 = e2 * state + dt * (e1 * k3)
     k4 = nonlin_fn(y4)
-    return e2 * state + (dt / 6.0) * (e2 * k1 + 2.0 * (e1 * (k2 + k3)) + k4)
+    acc = e2 * k1
+    acc = acc + 2.0 * (e1 * k2)
+    acc = acc + 2.0 * (e1 * k3)
+    return e2 * state + (dt / 6.0) * (acc + k4)
 
 
 # ---------------------------------------------------------------------------
@@ -440,9 +443,14 @@ class Solver:
         four stages."""
         u, v = self.compute_velocity(theta_hat_new)
         out = {}
+        g = self.grid
         for name, s_hat in scalars.items():
             kap = kappas.get(name, 0.0)
-            out[name] = rk4_step(s_hat, lambda s: scalar_rhs(s, u, v, self.grid, kap), dt)
+            if self.scheme == "if":
+                out[name] = if_rk4_step(
+                    s_hat, lambda s: fft2(-jacobian(ifft2(s), u, v, g)), -kap * g.k2, dt)
+            else:
+                out[name] = rk4_step(s_hat, lambda s: scalar_rhs(s, u, v, g, kap), dt)
         return out
 
     def get_diagnostics(self, state):

@@ -1,0 +1,24 @@
+"""pygsquig_amd: MI355X-native pseudo-spectral gSQG time step.
+
+Drop-in for pygSQuiG's hot path (``make_grid``, ``gSQGSolver.step``,
+``gSQGSolverOptimized.multistep``, ``gSQGSolverWithScalars``, ``RingForcing``,
+``CombinedDamping``) on PyTorch-ROCm device tensors, with the arithmetic in
+hand-written HIP kernels for gfx950 behind the C ABI of include/gsqg.h.
+"""
+
+from .grid import Grid, fft2, ifft2, make_grid
+from .forcing import CombinedDamping, RingForcing, large_scale_damping
+from .operators import (compute_streamfunction, compute_velocity_from_theta,
+                        fractional_laplacian, gradient, hyperviscosity, jacobian, laplacian,
+                        perpendicular_gradient)
+from .solver import gSQGSolver, gSQGSolverOptimized
+from ._lib import GsqgLibraryError
+
+__all__ = [
+    "Grid", "make_grid", "fft2", "ifft2", "RingForcing", "CombinedDamping",
+    "large_scale_damping", "gradient", "laplacian", "fractional_laplacian",
+    "perpendicular_gradient", "jacobian", "compute_streamfunction",
+    "compute_velocity_from_theta", "hyperviscosity", "gSQGSolver", "gSQGSolverOptimized",
+    "GsqgLibraryError",
+]
+__version__ = "0.1.0"

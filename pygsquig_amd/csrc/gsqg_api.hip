@@ -1,0 +1,629 @@
+// Host side of the C ABI declared in include/gsqg.h: plan set-up, device
+// buffers and the launch sequence of one RK4 step.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/gsqg.h"
+#include "gsqg_internal.h"
+
+namespace gsqg {
+hipError_t launch_ck(int N, int job, const CKArgs& a, int ny, hipStream_t s);
+hipError_t launch_rk(int N, int job, const RKArgs& a, int ny, hipStream_t s);
+int ck_num_blocks(int N);
+hipError_t launch_full_to_half(const double2* X, double2* H, int N, double* norms, double* norm0,
+                               hipStream_t s);
+int full_to_half_num_blocks(int N);
+hipError_t launch_half_to_full(const double2* H, double2* X, int N, hipStream_t s);
+hipError_t launch_a_gather(const double2* X, const int2* modes, double2* aval, int n, int N,
+                           double* norm, hipStream_t s);
+hipError_t launch_a_scatter(double2* X, const int2* modes, const double2* aval, int n, int N,
+                            hipStream_t s);
+hipError_t launch_a_rhs(const Phys& ph, const double* k1d, const uint8_t* flags, const Ctrl* ctrl,
+                        const int2* modes, const double2* aval, double2* out, int n, int N,
+                        hipStream_t s);
+hipError_t launch_set_ctrl(Ctrl* c, int64_t step, uint64_t key, uint64_t seed, int key_mode,
+                           hipStream_t s);
+}  // namespace gsqg
+
+using namespace gsqg;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(GSQG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+static const int kMaxScalars = 8;
+
+struct gsqg_plan {
+  int N = 0;
+  int device = 0;
+  size_t hsz = 0;  // complex elements of one half-packed field
+  Phys ph{};
+  int n_scalars = 0;
+  double kappa[kMaxScalars] = {0};
+  // theta
+  double2 *th = nullptr, *th2 = nullptr, *acc = nullptr, *ys = nullptr, *J = nullptr;
+  double2* F[4] = {nullptr, nullptr, nullptr, nullptr};
+  // scalars (contiguous, stride hsz)
+  double2 *s_th = nullptr, *s_acc = nullptr, *s_ys = nullptr, *s_J = nullptr;
+  double2 *s_F0 = nullptr, *s_F1 = nullptr;
+  double *uphys = nullptr, *vphys = nullptr;
+  // tables
+  double* k1d = nullptr;
+  uint8_t* flags = nullptr;
+  double2* tw = nullptr;
+  Ctrl* ctrl = nullptr;
+  double* part = nullptr;
+  int part_stride = 0;
+  int2* ring = nullptr;
+  double2* a_val = nullptr;
+  double2* a_tmp = nullptr;
+  int n_ring = 0;
+  double* norms = nullptr;
+  int n_norms = 0;
+  size_t bytes = 0;
+  // graph cache
+  hipStream_t cap = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  double g_dt = 0.0;
+  int g_key_mode = -1;
+  uint64_t g_key = 0;
+};
+
+template <typename T>
+static hipError_t dalloc(gsqg_plan* p, T** ptr, size_t count) {
+  size_t b = count * sizeof(T);
+  if (b == 0) b = 16;
+  hipError_t e = hipMalloc((void**)ptr, b);
+  if (e == hipSuccess) p->bytes += b;
+  return e;
+}
+
+static void free_plan(gsqg_plan* p) {
+  if (!p) return;
+  void* ptrs[] = {p->th,   p->th2,   p->acc,  p->ys,   p->J,     p->F[0], p->F[1], p->F[2],
+                  p->F[3], p->s_th,  p->s_acc, p->s_ys, p->s_J,  p->s_F0, p->s_F1, p->uphys,
+                  p->vphys, p->k1d,  p->flags, p->tw,   p->ctrl, p->part, p->ring, p->a_val,
+                  p->a_tmp, p->norms};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+  if (p->cap) (void)hipStreamDestroy(p->cap);
+  delete p;
+}
+
+extern "C" int gsqg_version(void) { return 100; }
+
+extern "C" const char* gsqg_last_error(void) { return g_err.c_str(); }
+
+static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+extern "C" int gsqg_plan_create(const gsqg_plan_desc* d, gsqg_plan** out) {
+  if (!d || !out) return fail(GSQG_ERR_ARG, "null argument");
+  *out = nullptr;
+  const int N = d->n;
+  if (N % 2 != 0) return fail(GSQG_ERR_ARG, "N must be even, got " + std::to_string(N));
+  if (!is_pow2(N) || N < 16 || N > 8192)
+    return fail(GSQG_ERR_UNSUPPORTED,
+                "this build supports power-of-two N in [16, 8192], got " + std::to_string(N));
+  if (!(d->alpha >= -2.0 && d->alpha <= 2.0))
+    return fail(GSQG_ERR_ARG, "alpha must be in [-2, 2]");
+  if (d->p != 2 && d->p != 4 && d->p != 8) return fail(GSQG_ERR_ARG, "p must be 2, 4, or 8");
+  if (d->damping && d->damp_p != 2 && d->damp_p != 4 && d->damp_p != 8)
+    return fail(GSQG_ERR_ARG, "damping p must be 2, 4, or 8");
+  if (d->n_scalars < 0 || d->n_scalars > kMaxScalars)
+    return fail(GSQG_ERR_UNSUPPORTED, "at most 8 passive scalars");
+  if (!d->k1d || !d->mode_flags) return fail(GSQG_ERR_ARG, "k1d and mode_flags are required");
+  if (d->forcing && d->n_ring > 0 && !d->ring_modes)
+    return fail(GSQG_ERR_ARG, "ring_modes required");
+
+  gsqg_plan* p = new gsqg_plan();
+  p->N = N;
+  HIPCHK(hipGetDevice(&p->device));
+  p->hsz = (size_t)(N / 2) * (size_t)N;
+  Phys& ph = p->ph;
+  ph.N = N;
+  ph.inv_n2 = 1.0 / ((double)N * (double)N);
+  ph.alpha = d->alpha;
+  ph.alpha_kind = d->alpha == 1.0 ? 1 : d->alpha == 0.0 ? 2 : d->alpha == 2.0 ? 3 : 0;
+  ph.nu = d->nu_p;
+  ph.p = d->p;
+  ph.damping = d->damping;
+  ph.mu = d->damp_mu;
+  ph.nu_d = d->damp_nu_p;
+  ph.p_d = d->damping ? d->damp_p : 2;
+  ph.forcing = d->forcing;
+  ph.amp = d->amplitude;
+  ph.eps = d->epsilon;
+  ph.scheme = d->scheme;
+  ph.combine = d->combine;
+  ph.hv_dc_zero = d->hv_dc_zero;
+  ph.kappa = 0.0;
+  p->n_scalars = d->n_scalars;
+  for (int s = 0; s < d->n_scalars; ++s) p->kappa[s] = d->kappa ? d->kappa[s] : 0.0;
+
+  hipError_t e = hipSuccess;
+#define ALLOC(ptr, cnt)                                        \
+  do {                                                         \
+    e = dalloc(p, &(ptr), (cnt));                              \
+    if (e != hipSuccess) {                                     \
+      free_plan(p);                                            \
+      return fail(GSQG_ERR_HIP, std::string("hipMalloc: ") +   \
+                                    hipGetErrorString(e));     \
+    }                                                          \
+  } while (0)
+  const size_t h = p->hsz;
+  ALLOC(p->th, h);
+  ALLOC(p->th2, h);
+  ALLOC(p->acc, h);
+  ALLOC(p->ys, h);
+  ALLOC(p->J, h);
+  for (int q = 0; q < 4; ++q) ALLOC(p->F[q], h);
+  if (p->n_scalars > 0) {
+    const size_t hs = h * p->n_scalars;
+    ALLOC(p->s_th, hs);
+    ALLOC(p->s_acc, hs);
+    ALLOC(p->s_ys, hs);
+    ALLOC(p->s_J, hs);
+    ALLOC(p->s_F0, hs);
+    ALLOC(p->s_F1, hs);
+  }
+  ALLOC(p->uphys, (size_t)N * N);
+  ALLOC(p->vphys, (size_t)N * N);
+  ALLOC(p->k1d, (size_t)N);
+  ALLOC(p->flags, h);
+  ALLOC(p->tw, (size_t)2 * 8192);
+  ALLOC(p->ctrl, 1);
+  p->part_stride = ck_num_blocks(N);
+  ALLOC(p->part, (size_t)8 * p->part_stride);
+  p->n_ring = d->forcing ? d->n_ring : 0;
+  ALLOC(p->ring, (size_t)(p->n_ring > 0 ? p->n_ring : 1));
+  ALLOC(p->a_val, (size_t)(p->n_ring > 0 ? p->n_ring : 1));
+  ALLOC(p->a_tmp, (size_t)(p->n_ring > 0 ? p->n_ring : 1));
+  p->n_norms = full_to_half_num_blocks(N) + 2;
+  ALLOC(p->norms, (size_t)p->n_norms);
+#undef ALLOC
+
+  // Twiddle tables tw[T + k] = exp(-2 pi i k / T), long double on the host.
+  std::vector<double2> tw(2 * 8192);
+  tw[0].x = tw[0].y = tw[1].x = tw[1].y = 0.0;
+  for (int T = 1; T <= 8192; T *= 2) {
+    for (int k = 0; k < T; ++k) {
+      long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / T;
+      tw[T + k].x = (double)cosl(ang);
+      tw[T + k].y = (double)sinl(ang);
+    }
+  }
+  HIPCHK(hipMemcpy(p->tw, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p->k1d, d->k1d, (size_t)N * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p->flags, d->mode_flags, h, hipMemcpyHostToDevice));
+  if (p->n_ring > 0) {
+    HIPCHK(hipMemcpy(p->ring, d->ring_modes, (size_t)p->n_ring * sizeof(int2),
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(p->a_val, 0, (size_t)p->n_ring * sizeof(double2)));
+  }
+  Ctrl c0{};
+  for (int i = 0; i < 8; ++i) c0.scale[i] = 1.0;
+  HIPCHK(hipMemcpy(p->ctrl, &c0, sizeof(Ctrl), hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(p->th, 0, h * sizeof(double2)));
+  if (p->n_scalars > 0) HIPCHK(hipMemset(p->s_th, 0, h * p->n_scalars * sizeof(double2)));
+  HIPCHK(hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking));
+  HIPCHK(hipDeviceSynchronize());
+  *out = p;
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_plan_destroy(gsqg_plan* p) {
+  if (!p) return GSQG_OK;
+  (void)hipDeviceSynchronize();
+  free_plan(p);
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_plan_device_bytes(const gsqg_plan* p, size_t* bytes) {
+  if (!p || !bytes) return fail(GSQG_ERR_ARG, "null argument");
+  *bytes = p->bytes;
+  return GSQG_OK;
+}
+
+// ----------------------------------------------------------- arg builders
+static CKArgs ck_theta(gsqg_plan* p, double2* th, int e, int prologue, int rate_slot,
+                       int rate_next, double dt) {
+  CKArgs a{};
+  a.J = p->J;
+  a.th = th;
+  a.acc = p->acc;
+  a.ys = p->ys;
+  for (int q = 0; q < 4; ++q) a.F[q] = p->F[q];
+  a.flags = p->flags;
+  a.k1d = p->k1d;
+  a.tw = p->tw;
+  a.ctrl = p->ctrl;
+  a.part = p->part;
+  a.part_stride = p->part_stride;
+  a.e = e;
+  a.prologue = prologue;
+  a.rate_slot = p->ph.forcing ? rate_slot : 0;
+  a.rate_next_key = rate_next;
+  a.dt = dt;
+  a.ph = p->ph;
+  a.a_modes = p->ring;
+  a.a_val = p->a_val;
+  a.n_a = p->n_ring;
+  a.field_stride = 0;
+  return a;
+}
+
+static RKArgs rk_theta(gsqg_plan* p, int red_slot, int inc_step, bool uv_out) {
+  RKArgs a{};
+  for (int q = 0; q < 4; ++q) a.F[q] = p->F[q];
+  a.uout = uv_out ? p->uphys : nullptr;
+  a.vout = uv_out ? p->vphys : nullptr;
+  a.Jout = p->J;
+  a.tw = p->tw;
+  a.ctrl = p->ctrl;
+  a.part = p->part;
+  a.part_stride = p->part_stride;
+  a.nblk_ck = p->part_stride;
+  a.red_slot = p->ph.forcing ? red_slot : 0;
+  a.inc_step = inc_step;
+  a.eps = p->ph.eps;
+  a.field_stride = 0;
+  return a;
+}
+
+// The scalar kernels read kappa from ph.kappa; scalars with different kappa
+// are launched one y-slice at a time when needed.
+static bool uniform_kappa(const gsqg_plan* p) {
+  for (int s = 1; s < p->n_scalars; ++s)
+    if (p->kappa[s] != p->kappa[0]) return false;
+  return true;
+}
+
+static int launch_scalar_ck(gsqg_plan* p, int e, int prologue, double dt, hipStream_t st) {
+  CKArgs a{};
+  a.flags = p->flags;
+  a.k1d = p->k1d;
+  a.tw = p->tw;
+  a.ctrl = p->ctrl;
+  a.part = p->part;
+  a.part_stride = p->part_stride;
+  a.e = e;
+  a.prologue = prologue;
+  a.dt = dt;
+  a.ph = p->ph;
+  a.ph.forcing = 0;
+  a.ph.damping = 0;
+  a.field_stride = (long long)p->hsz;
+  const bool uni = uniform_kappa(p);
+  const int groups = uni ? 1 : p->n_scalars;
+  for (int g = 0; g < groups; ++g) {
+    const size_t off = uni ? 0 : (size_t)g * p->hsz;
+    a.J = p->s_J + off;
+    a.th = p->s_th + off;
+    a.acc = p->s_acc + off;
+    a.ys = p->s_ys + off;
+    a.F[0] = p->s_F0 + off;
+    a.F[1] = p->s_F1 + off;
+    a.F[2] = a.F[3] = nullptr;
+    a.ph.kappa = p->kappa[uni ? 0 : g];
+    HIPCHK(launch_ck(p->N, CK_SCALAR, a, uni ? p->n_scalars : 1, st));
+  }
+  return GSQG_OK;
+}
+
+static int launch_scalar_rk(gsqg_plan* p, const double* u, const double* v, hipStream_t st) {
+  RKArgs a{};
+  a.F[0] = p->s_F0;
+  a.F[1] = p->s_F1;
+  a.uphys = u;
+  a.vphys = v;
+  a.Jout = p->s_J;
+  a.tw = p->tw;
+  a.ctrl = p->ctrl;
+  a.part = p->part;
+  a.field_stride = (long long)p->hsz;
+  HIPCHK(launch_rk(p->N, RK_SCALAR, a, p->n_scalars, st));
+  return GSQG_OK;
+}
+
+// One scalar RK4 step with the frozen (u, v) in uphys/vphys; the scalar
+// prologue fields of the current state must already be in s_F0/s_F1.
+static int scalar_step(gsqg_plan* p, double dt, bool next_prologue, hipStream_t st,
+                       const double* u = nullptr, const double* v = nullptr) {
+  int rc;
+  if (!u) u = p->uphys;
+  if (!v) v = p->vphys;
+  for (int e = 1; e <= 4; ++e) {
+    if ((rc = launch_scalar_rk(p, u, v, st))) return rc;
+    if ((rc = launch_scalar_ck(p, e, (e < 4) ? 1 : (next_prologue ? 1 : 0), dt, st))) return rc;
+  }
+  return GSQG_OK;
+}
+
+// Theta stages 2..4 and the final combine of one step (stage 1's RK is
+// issued by the caller so that the scalar step can slot in after it).
+static int theta_rest(gsqg_plan* p, double dt, bool derived, bool last_prologue, int last_rate,
+                      hipStream_t st) {
+  const int N = p->N;
+  for (int e = 1; e <= 3; ++e) {
+    HIPCHK(launch_ck(N, CK_THETA, ck_theta(p, p->th, e, 1, e + 1, 0, dt), 1, st));
+    HIPCHK(launch_rk(N, RK_THETA, rk_theta(p, e + 1, 0, false), 1, st));
+  }
+  HIPCHK(launch_ck(N, CK_THETA,
+                   ck_theta(p, p->th, 4, last_prologue ? 1 : 0, last_rate, derived ? 1 : 0, dt),
+                   1, st));
+  return GSQG_OK;
+}
+
+// Steady-state step body: theta step s (s >= 1) and, if scalars exist, the
+// scalar step s-1 slotted after theta's stage-1 row pass.
+static int step_body(gsqg_plan* p, double dt, bool derived, bool scalars_pending,
+                     bool last_prologue, int last_rate, bool scalar_next_prologue,
+                     hipStream_t st) {
+  const int N = p->N;
+  HIPCHK(launch_rk(N, RK_THETA, rk_theta(p, 1, derived ? 1 : 0, scalars_pending), 1, st));
+  int rc;
+  if (scalars_pending) {
+    if ((rc = scalar_step(p, dt, scalar_next_prologue, st))) return rc;
+  }
+  return theta_rest(p, dt, derived, last_prologue, last_rate, st);
+}
+
+extern "C" int gsqg_advance(gsqg_plan* p, double dt, int n_steps, int key_mode, uint64_t key,
+                            int64_t step0, int use_graph, void* stream) {
+  if (!p) return fail(GSQG_ERR_ARG, "null plan");
+  if (n_steps < 0) return fail(GSQG_ERR_ARG, "n_steps must be >= 0");
+  if (n_steps == 0) return GSQG_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const int N = p->N;
+  const bool derived = key_mode == GSQG_KEY_DERIVED;
+  const bool sc = p->n_scalars > 0;
+  int rc;
+  HIPCHK(launch_set_ctrl(p->ctrl, derived ? step0 - 1 : step0, key, key, derived ? 1 : 0, st));
+  // Prologue of the first stage.
+  HIPCHK(launch_ck(N, CK_THETA, ck_theta(p, p->th, 0, 1, 1, derived ? 1 : 0, dt), 1, st));
+  if (sc) {
+    if ((rc = launch_scalar_ck(p, 0, 1, dt, st))) return rc;
+  }
+  // step 0
+  {
+    const bool last = (n_steps == 1);
+    HIPCHK(launch_rk(N, RK_THETA, rk_theta(p, 1, derived ? 1 : 0, false), 1, st));
+    if ((rc = theta_rest(p, dt, derived, !last || sc, last ? 0 : 1, st))) return rc;
+  }
+  // middle steps 1 .. n-2
+  const int n_mid = n_steps >= 2 ? n_steps - 2 : 0;
+  if (n_mid > 0) {
+    if (use_graph && n_mid >= 2) {
+      if (!p->gexec || p->g_dt != dt || p->g_key_mode != key_mode || p->g_key != key) {
+        if (p->gexec) {
+          HIPCHK(hipGraphExecDestroy(p->gexec));
+          p->gexec = nullptr;
+        }
+        hipGraph_t g;
+        HIPCHK(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
+        rc = step_body(p, dt, derived, sc, true, 1, true, p->cap);
+        hipError_t ce = hipStreamEndCapture(p->cap, &g);
+        if (rc) return rc;
+        if (ce != hipSuccess) return fail(GSQG_ERR_HIP, std::string("capture: ") + hipGetErrorString(ce));
+        HIPCHK(hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0));
+        HIPCHK(hipGraphDestroy(g));
+        p->g_dt = dt;
+        p->g_key_mode = key_mode;
+        p->g_key = key;
+      }
+      for (int s = 0; s < n_mid; ++s) HIPCHK(hipGraphLaunch(p->gexec, st));
+    } else {
+      for (int s = 0; s < n_mid; ++s) {
+        if ((rc = step_body(p, dt, derived, sc, true, 1, true, st))) return rc;
+      }
+    }
+  }
+  // last step (if n >= 2)
+  if (n_steps >= 2) {
+    if ((rc = step_body(p, dt, derived, sc, sc, 0, true, st))) return rc;
+  }
+  if (sc) {
+    // velocity of the final theta (prologue fields F2, F3 of the last CK)
+    RKArgs a{};
+    a.F[0] = p->F[2];
+    a.F[1] = p->F[3];
+    a.uout = p->uphys;
+    a.vout = p->vphys;
+    a.tw = p->tw;
+    a.ctrl = p->ctrl;
+    HIPCHK(launch_rk(N, RK_VEL, a, 1, st));
+    if ((rc = scalar_step(p, dt, false, st))) return rc;
+  }
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_advance_scalars(gsqg_plan* p, const double* u, const double* v, double dt,
+                                    void* stream) {
+  if (!p || !u || !v) return fail(GSQG_ERR_ARG, "null argument");
+  if (p->n_scalars == 0) return fail(GSQG_ERR_ARG, "plan has no passive scalars");
+  hipStream_t st = (hipStream_t)stream;
+  int rc;
+  if ((rc = launch_scalar_ck(p, 0, 1, dt, st))) return rc;
+  return scalar_step(p, dt, false, st, u, v);
+}
+
+// ------------------------------------------------------------ state I/O
+static int load_half(gsqg_plan* p, const double2* X, double2* H, double* offsq, bool ring,
+                     hipStream_t st) {
+  const int N = p->N;
+  const int nb = full_to_half_num_blocks(N);
+  HIPCHK(launch_full_to_half(X, H, N, p->norms, p->norms + nb, st));
+  if (ring && p->n_ring > 0)
+    HIPCHK(launch_a_gather(X, p->ring, p->a_val, p->n_ring, N, p->norms + nb + 1, st));
+  if (offsq) {
+    std::vector<double> h(p->n_norms, 0.0);
+    HIPCHK(hipMemcpyAsync(h.data(), p->norms, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          st));
+    HIPCHK(hipStreamSynchronize(st));
+    double tot = 0.0;
+    for (int i = 0; i <= nb; ++i) tot += h[i];
+    double ringsq = (ring && p->n_ring > 0) ? h[nb + 1] : 0.0;
+    double off = tot - ringsq;
+    *offsq = off > 0.0 ? off : 0.0;
+  }
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_load_theta(gsqg_plan* p, const void* X, double* offring_sq, void* stream) {
+  if (!p || !X) return fail(GSQG_ERR_ARG, "null argument");
+  return load_half(p, (const double2*)X, p->th, offring_sq, true, (hipStream_t)stream);
+}
+
+extern "C" int gsqg_store_theta(gsqg_plan* p, void* X, void* stream) {
+  if (!p || !X) return fail(GSQG_ERR_ARG, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(launch_half_to_full(p->th, (double2*)X, p->N, st));
+  if (p->n_ring > 0) HIPCHK(launch_a_scatter((double2*)X, p->ring, p->a_val, p->n_ring, p->N, st));
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_load_scalar(gsqg_plan* p, int idx, const void* X, double* offsq,
+                                void* stream) {
+  if (!p || !X) return fail(GSQG_ERR_ARG, "null argument");
+  if (idx < 0 || idx >= p->n_scalars) return fail(GSQG_ERR_ARG, "scalar index out of range");
+  return load_half(p, (const double2*)X, p->s_th + (size_t)idx * p->hsz, offsq, false,
+                   (hipStream_t)stream);
+}
+
+extern "C" int gsqg_store_scalar(gsqg_plan* p, int idx, void* X, void* stream) {
+  if (!p || !X) return fail(GSQG_ERR_ARG, "null argument");
+  if (idx < 0 || idx >= p->n_scalars) return fail(GSQG_ERR_ARG, "scalar index out of range");
+  HIPCHK(launch_half_to_full(p->s_th + (size_t)idx * p->hsz, (double2*)X, p->N,
+                             (hipStream_t)stream));
+  return GSQG_OK;
+}
+
+// ------------------------------------------------------------- operators
+extern "C" int gsqg_rhs(gsqg_plan* p, const void* X, void* out, uint64_t key, void* stream) {
+  if (!p || !X || !out) return fail(GSQG_ERR_ARG, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const int N = p->N;
+  const int nb = full_to_half_num_blocks(N);
+  HIPCHK(launch_full_to_half((const double2*)X, p->th2, N, p->norms, p->norms + nb, st));
+  if (p->n_ring > 0)
+    HIPCHK(launch_a_gather((const double2*)X, p->ring, p->a_tmp, p->n_ring, N, nullptr, st));
+  HIPCHK(launch_set_ctrl(p->ctrl, 0, key, key, 0, st));
+  // compute_rhs is the explicit right-hand side whatever the plan's scheme.
+  CKArgs c0 = ck_theta(p, p->th2, 0, 1, 1, 0, 0.0);
+  c0.ph.scheme = 0;
+  HIPCHK(launch_ck(N, CK_THETA, c0, 1, st));
+  HIPCHK(launch_rk(N, RK_THETA, rk_theta(p, 1, 0, false), 1, st));
+  // Stage-1 epilogue writes k1 into acc (th2 is left untouched for e == 1).
+  CKArgs c1 = ck_theta(p, p->th2, 1, 0, 0, 0, 0.0);
+  c1.ph.scheme = 0;
+  HIPCHK(launch_ck(N, CK_THETA, c1, 1, st));
+  HIPCHK(launch_half_to_full(p->acc, (double2*)out, N, st));
+  if (p->n_ring > 0) {
+    HIPCHK(launch_a_rhs(p->ph, p->k1d, p->flags, p->ctrl, p->ring, p->a_tmp, p->a_tmp, p->n_ring,
+                        N, st));
+    HIPCHK(launch_a_scatter((double2*)out, p->ring, p->a_tmp, p->n_ring, N, st));
+  }
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_velocity(gsqg_plan* p, const void* X, double* u, double* v, void* stream) {
+  if (!p || !X || !u || !v) return fail(GSQG_ERR_ARG, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const int N = p->N;
+  const int nb = full_to_half_num_blocks(N);
+  HIPCHK(launch_full_to_half((const double2*)X, p->th2, N, p->norms, p->norms + nb, st));
+  CKArgs a = ck_theta(p, p->th2, 0, 1, 0, 0, 0.0);
+  a.F[0] = p->F[2];
+  a.F[1] = p->F[3];
+  HIPCHK(launch_ck(N, CK_VEL, a, 1, st));
+  RKArgs r{};
+  r.F[0] = p->F[2];
+  r.F[1] = p->F[3];
+  r.uout = u;
+  r.vout = v;
+  r.tw = p->tw;
+  r.ctrl = p->ctrl;
+  HIPCHK(launch_rk(N, RK_VEL, r, 1, st));
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_fft2(gsqg_plan* p, const double* f, void* X, void* stream) {
+  if (!p || !f || !X) return fail(GSQG_ERR_ARG, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const int N = p->N;
+  RKArgs r{};
+  r.phys_in = f;
+  r.Jout = p->J;
+  r.tw = p->tw;
+  r.ctrl = p->ctrl;
+  HIPCHK(launch_rk(N, RK_R2C, r, 1, st));
+  CKArgs a = ck_theta(p, p->th2, 0, 0, 0, 0, 0.0);
+  a.J = p->J;
+  a.th = p->th2;
+  HIPCHK(launch_ck(N, CK_FWD, a, 1, st));
+  HIPCHK(launch_half_to_full(p->th2, (double2*)X, N, st));
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_ifft2(gsqg_plan* p, const void* X, double* f, void* stream) {
+  if (!p || !X || !f) return fail(GSQG_ERR_ARG, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const int N = p->N;
+  const int nb = full_to_half_num_blocks(N);
+  HIPCHK(launch_full_to_half((const double2*)X, p->th2, N, p->norms, p->norms + nb, st));
+  CKArgs a = ck_theta(p, p->th2, 0, 0, 0, 0, 0.0);
+  a.th = p->th2;
+  a.F[0] = p->F[0];
+  HIPCHK(launch_ck(N, CK_INV, a, 1, st));
+  RKArgs r{};
+  r.F[0] = p->F[0];
+  r.phys_out = f;
+  r.tw = p->tw;
+  r.ctrl = p->ctrl;
+  HIPCHK(launch_rk(N, RK_C2R, r, 1, st));
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_nonfinite(gsqg_plan* p, int* flag, int clear, void* stream) {
+  if (!p || !flag) return fail(GSQG_ERR_ARG, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  Ctrl c;
+  HIPCHK(hipMemcpyAsync(&c, p->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  *flag = c.nonfinite != 0.0;
+  if (clear && *flag) {
+    double z = 0.0;
+    HIPCHK(hipMemcpyAsync((char*)p->ctrl + offsetof(Ctrl, nonfinite), &z, sizeof(double),
+                          hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_forcing_stats(gsqg_plan* p, double* rate4, double* scale4, void* stream) {
+  if (!p) return fail(GSQG_ERR_ARG, "null plan");
+  hipStream_t st = (hipStream_t)stream;
+  Ctrl c;
+  HIPCHK(hipMemcpyAsync(&c, p->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int i = 0; i < 4; ++i) {
+    if (rate4) rate4[i] = c.rate[i + 1];
+    if (scale4) scale4[i] = c.scale[i + 1];
+  }
+  return GSQG_OK;
+}

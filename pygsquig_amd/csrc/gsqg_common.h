@@ -1,0 +1,194 @@
+// Device-side building blocks shared by every gSQG kernel (gfx950, wave64).
+//
+// Complex FP64 helpers, the SplitMix64 phase hash used for ring forcing, and a
+// Stockham radix-8 FFT that runs across L/8 lanes per transform with each
+// lane holding 8 elements in registers and exchanging through LDS between
+// passes.  Element ownership is the same on entry and exit: lane t of a
+// transform holds elements t + m*L/8, m = 0..7, in natural order, so the first
+// pass reads registers and the last pass writes registers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gsqg {
+
+using cd = double2;
+
+__device__ __forceinline__ cd cmk(double a, double b) {
+  cd r;
+  r.x = a;
+  r.y = b;
+  return r;
+}
+__device__ __forceinline__ cd cadd(cd a, cd b) { return cmk(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ cd csub(cd a, cd b) { return cmk(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ cd cmul(cd a, cd b) {
+  return cmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__device__ __forceinline__ cd cmulc(cd a, cd b) {
+  return cmk(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ cd cconj(cd a) { return cmk(a.x, -a.y); }
+__device__ __forceinline__ cd cscale(cd a, double s) { return cmk(a.x * s, a.y * s); }
+__device__ __forceinline__ cd cmul_i(cd a) { return cmk(-a.y, a.x); }    // i*a
+__device__ __forceinline__ cd cmul_mi(cd a) { return cmk(a.y, -a.x); }   // -i*a
+
+// Multiply by DIR*i (DIR = -1: forward transform sign, +1: inverse).
+template <int DIR>
+__device__ __forceinline__ cd rot90(cd a) {
+  return DIR < 0 ? cmk(a.y, -a.x) : cmk(-a.y, a.x);
+}
+
+// ---------------------------------------------------------------- hashing
+// SplitMix64 output function (Steele/Lea/Flood 2014, constants of Vigna's
+// splitmix64.c).  Bit-identical to oracle/rng.py.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ uint64_t phase_bits(uint64_t key, uint64_t idx) {
+  return mix64(key + (idx + 1ull) * 0x9E3779B97F4A7C15ull);
+}
+__device__ __forceinline__ uint64_t step_key(uint64_t seed, int64_t n) {
+  return mix64(seed + (uint64_t)(n + 1) * 0xD1B54A32D192ED03ull);
+}
+// phi = u * 2pi with u = (bits >> 11) * 2^-53.
+__device__ __forceinline__ double phase_of(uint64_t key, uint64_t idx) {
+  const double two_pi = 6.283185307179586;  // binary64 nearest to 2*pi
+  double u = (double)(phase_bits(key, idx) >> 11) * 0x1.0p-53;
+  return u * two_pi;
+}
+
+// ---------------------------------------------------------------- FFT
+// LDS index with one pad element per 8 to break the stride-8 bank pattern of
+// the first Stockham pass.
+__device__ __forceinline__ int lpad(int e) { return e + (e >> 3); }
+template <int L>
+struct Padded {
+  static constexpr int value = L + L / 8;
+};
+
+template <int DIR>
+__device__ __forceinline__ void dft2(cd& a, cd& b) {
+  cd t = a;
+  a = cadd(t, b);
+  b = csub(t, b);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft4(cd& a0, cd& a1, cd& a2, cd& a3) {
+  cd s02 = cadd(a0, a2), d02 = csub(a0, a2);
+  cd s13 = cadd(a1, a3), d13 = rot90<DIR>(csub(a1, a3));
+  a0 = cadd(s02, s13);
+  a2 = csub(s02, s13);
+  a1 = cadd(d02, d13);
+  a3 = csub(d02, d13);
+}
+
+// In-register 8-point DFT, natural order in and out.
+template <int DIR>
+__device__ __forceinline__ void dft8(cd (&v)[8]) {
+  const double r2 = 0.70710678118654752440;  // 1/sqrt(2)
+  cd e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+  cd o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+  dft4<DIR>(e0, e1, e2, e3);
+  dft4<DIR>(o0, o1, o2, o3);
+  // twiddles W8^k, W8 = exp(DIR * i*pi/4)
+  cd t1, t3;
+  if (DIR < 0) {
+    t1 = cmk((o1.x + o1.y) * r2, (o1.y - o1.x) * r2);      // (1-i)/sqrt2
+    t3 = cmk((-o3.x + o3.y) * r2, (-o3.y - o3.x) * r2);    // (-1-i)/sqrt2
+  } else {
+    t1 = cmk((o1.x - o1.y) * r2, (o1.y + o1.x) * r2);      // (1+i)/sqrt2
+    t3 = cmk((-o3.x - o3.y) * r2, (o3.x - o3.y) * r2);     // (-1+i)/sqrt2
+  }
+  cd t2 = rot90<DIR>(o2);
+  v[0] = cadd(e0, o0);
+  v[4] = csub(e0, o0);
+  v[1] = cadd(e1, t1);
+  v[5] = csub(e1, t1);
+  v[2] = cadd(e2, t2);
+  v[6] = csub(e2, t2);
+  v[3] = cadd(e3, t3);
+  v[7] = csub(e3, t3);
+}
+
+// Twiddle tables: tw[T + k] = exp(-2 pi i k / T) for every power of two T.
+template <int DIR>
+__device__ __forceinline__ cd twiddle(const cd* __restrict__ tw, int T, int k) {
+  cd w = tw[T + k];
+  return DIR < 0 ? w : cconj(w);
+}
+
+// One length-L transform over L/8 lanes (lane index t in [0, L/8)); `buf` is
+// this transform's LDS region of Padded<L> elements.  All lanes of the block
+// must call it together (it contains block barriers).
+template <int L, int DIR>
+__device__ __forceinline__ void fft_lanes(cd (&v)[8], cd* buf, int t, const cd* __restrict__ tw) {
+  static_assert(L >= 8 && (L & (L - 1)) == 0, "power of two >= 8");
+  constexpr int Q = L / 8;
+  constexpr int LOG = __builtin_ctz(L);
+  constexpr int P8 = LOG / 3;
+  constexpr int REM = LOG % 3;
+  int Ns = 1;
+#pragma unroll
+  for (int p = 0; p < P8; ++p) {
+    if (p > 0) {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) v[m] = buf[lpad(t + m * Q)];
+    }
+    if (Ns > 1) {
+      int jm = t % Ns;
+#pragma unroll
+      for (int r = 1; r < 8; ++r) v[r] = cmul(v[r], twiddle<DIR>(tw, Ns * 8, r * jm));
+    }
+    dft8<DIR>(v);
+    bool last = (p == P8 - 1) && (REM == 0);
+    if (!last) {
+      __syncthreads();
+      int base = (t / Ns) * Ns * 8 + (t % Ns);
+#pragma unroll
+      for (int r = 0; r < 8; ++r) buf[lpad(base + r * Ns)] = v[r];
+      __syncthreads();
+    }
+    Ns *= 8;
+  }
+  if (REM == 1) {  // radix-2 final pass, Ns = L/2, 4 butterflies per lane
+    if (P8 > 0) {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) v[m] = buf[lpad(t + m * Q)];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int j = t + q * Q;
+      cd a = v[q], b = v[q + 4];
+      b = cmul(b, twiddle<DIR>(tw, L, j));
+      dft2<DIR>(a, b);
+      v[q] = a;
+      v[q + 4] = b;
+    }
+  } else if (REM == 2) {  // radix-4 final pass, Ns = L/4, 2 butterflies per lane
+    if (P8 > 0) {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) v[m] = buf[lpad(t + m * Q)];
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      int j = t + q * Q;
+      cd a0 = v[q], a1 = v[q + 2], a2 = v[q + 4], a3 = v[q + 6];
+      a1 = cmul(a1, twiddle<DIR>(tw, L, j));
+      a2 = cmul(a2, twiddle<DIR>(tw, L, 2 * j));
+      a3 = cmul(a3, twiddle<DIR>(tw, L, 3 * j));
+      dft4<DIR>(a0, a1, a2, a3);
+      v[q] = a0;
+      v[q + 2] = a1;
+      v[q + 4] = a2;
+      v[q + 6] = a3;
+    }
+  }
+}
+
+}  // namespace gsqg

@@ -1,0 +1,102 @@
+// Argument blocks shared by the host launchers and the kernels.
+//
+// Spectral layout ("half-packed, ky-major"): a field is N/2 rows of N complex
+// doubles, row j = ky index (0..N/2-1), column i = kx index in FFT order.
+// Row 0 packs the two self-conjugate columns of the reference's full (N, N)
+// spectrum: H[0][i] = X(i, 0) + i * X(i, N/2).  Mixed fields (x-inverse
+// transformed) use the same shape with column i = x; physical fields are
+// (N, N) doubles, row x, column y (the reference's C order).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gsqg {
+
+enum FlagBits : unsigned {
+  FLAG_DEALIAS = 1u,  // |k| < (2/3) pi N / L            (core/grid.py:102-105)
+  FLAG_LSDAMP = 2u,   // |k| < kf/2                      (forcing/damping.py:34)
+  FLAG_RING = 4u,     // ||k| - kf| <= dk/2              (forcing/ring_forcing.py:74-75)
+};
+// Row 0 of the flag array: low nibble = mode (i, 0), high nibble = (i, N/2).
+
+// Device control block, one per plan.
+struct Ctrl {
+  double scale[8];   // ring-forcing scale per stage slot (1..4)
+  double rate[8];    // <theta f> per stage slot
+  uint64_t key;      // explicit per-call forcing key
+  uint64_t seed;     // seed of derived per-step keys
+  int64_t step;      // index of the step being taken
+  int key_mode;      // 0: explicit key, 1: step_key(seed, step)
+  int pad_;
+  double nonfinite;  // set to 1 by any kernel that writes a non-finite state
+};
+
+struct Phys {
+  int N;
+  double inv_n2;
+  double alpha;
+  int alpha_kind;   // 0 generic pow, 1 alpha==1, 2 alpha==0, 3 alpha==2
+  double nu;        // solver hyperviscosity
+  int p;
+  int damping;      // CombinedDamping present
+  double mu;
+  double nu_d;
+  int p_d;
+  int forcing;      // RingForcing present
+  double amp;       // forcing amplitude
+  double eps;       // target <theta F>
+  int scheme;       // 0 explicit (reference), 1 integrating factor
+  int combine;      // 0 dt*(k1+2k2+2k3+k4)/6, 1 (dt/6)*(...)
+  int hv_dc_zero;   // 1: hyperviscosity zeroed at k=0 (reference gSQGSolver)
+  double kappa;     // scalar diffusivity
+};
+
+enum CKJob { CK_THETA = 0, CK_SCALAR = 1, CK_VEL = 2, CK_FWD = 3, CK_INV = 4 };
+enum RKJob { RK_THETA = 0, RK_SCALAR = 1, RK_VEL = 2, RK_R2C = 3, RK_C2R = 4 };
+
+struct CKArgs {
+  const double2* J;      // mixed input (forward x-transform in the epilogue)
+  double2* th;
+  double2* acc;
+  double2* ys;
+  double2* F[4];         // prologue outputs (mixed)
+  const uint8_t* flags;
+  const double* k1d;
+  const double2* tw;
+  Ctrl* ctrl;
+  double* part;          // rate partials, [8][part_stride]
+  int part_stride;
+  int e;                 // epilogue stage (0 = none)
+  int prologue;          // write prologue fields
+  int rate_slot;         // slot of the prologue state's <theta f> partials (0 = none)
+  int rate_next_key;     // prologue rate uses the key of step+1
+  double dt;
+  Phys ph;
+  const int2* a_modes;   // anti-Hermitian ring side state: (i, jy) of each entry
+  double2* a_val;
+  int n_a;
+  long long field_stride; // elements between consecutive scalars (blockIdx.y)
+};
+
+struct RKArgs {
+  const double2* F[4];
+  const double* uphys;
+  const double* vphys;
+  double* uout;
+  double* vout;
+  double2* Jout;
+  const double* phys_in;
+  double* phys_out;
+  const double2* tw;
+  Ctrl* ctrl;
+  const double* part;
+  int part_stride;
+  int nblk_ck;           // number of partials to reduce
+  int red_slot;          // stage slot to reduce (0 = none)
+  int inc_step;          // block 0 increments ctrl->step
+  double eps;
+  long long field_stride;
+};
+
+}  // namespace gsqg

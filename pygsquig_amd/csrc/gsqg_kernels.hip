@@ -1,0 +1,1069 @@
+// Fused pseudo-spectral RK4 kernels for the gSQG step on gfx950.
+//
+// One RHS stage of the reference (pygsquig/core/solver.py:28-58: 9 c2c FFTs
+// over full (N, N) complex arrays) becomes two launches over the half-packed
+// ky-major layout described in gsqg_internal.h:
+//
+//   column kernel  (ck_kernel, x direction, one block per N/8-lane row group)
+//     epilogue of stage e:  forward x-FFT of the y-transformed Jacobian,
+//                           dealias, hyperviscosity, damping, ring forcing,
+//                           RK4 (or integrating-factor) combination
+//     prologue of stage e+1: Hermitianised multipliers i kx, i ky,
+//                           -i ky |k|^-a, i kx |k|^-a (x 1/N^2), inverse
+//                           x-FFT of the four fields; ring <theta f> partials
+//   row kernel     (rk_kernel, y direction, one block per WX x-rows)
+//     four C2R y-transforms (half-length complex FFT + pre-twist), the
+//     product u*theta_x + v*theta_y in registers, one R2C y-transform.
+//
+// Both kernels keep each transform in LDS/registers: a field crosses HBM
+// exactly once per read or write.
+#include "gsqg_common.h"
+#include "gsqg_internal.h"
+
+namespace gsqg {
+
+// ------------------------------------------------------------------ shapes
+template <int N>
+struct CKShape {
+  static constexpr int Q = N / 8;  // lanes per row
+  static constexpr int R0 = (128 / Q) > 1 ? (128 / Q) : 1;
+  static constexpr int RPW = R0 < (N / 2) ? R0 : (N / 2);  // rows per block
+  static constexpr int NT = RPW * Q;
+  static constexpr int PADL = Padded<N>::value;
+};
+
+template <int N>
+struct RKShape {
+  static constexpr int M = N / 2;
+  static constexpr int Q = M / 8;  // lanes per x-row
+  static constexpr int W0 = (128 / Q) > 1 ? (128 / Q) : 1;
+  static constexpr int WX = W0 < N ? W0 : N;  // x-rows per block
+  static constexpr int NT = WX * Q;
+  static constexpr int PADM = Padded<M>::value;
+};
+
+// ------------------------------------------------------- per-mode physics
+__device__ __forceinline__ double hv_factor(double k2, int p) {
+  if (p == 2) return k2;
+  double s = k2 * k2;
+  if (p == 4) return s;
+  return s * s;
+}
+
+// |k|^-alpha, zero at k = 0 (core/operators.py:149-153).
+__device__ __forceinline__ double inv_frac(double k2, const Phys& ph) {
+  if (!(k2 > 0.0)) return 0.0;
+  switch (ph.alpha_kind) {
+    case 1:
+      return 1.0 / sqrt(k2);
+    case 2:
+      return 1.0;
+    case 3:
+      return 1.0 / k2;
+    default:
+      return pow(sqrt(k2), -ph.alpha);
+  }
+}
+
+// Real diagonal linear operator: solver hyperviscosity plus CombinedDamping.
+__device__ __forceinline__ double lin_coeff(double k2, bool dc, unsigned fl, const Phys& ph) {
+  double l = (dc && ph.hv_dc_zero) ? 0.0 : -ph.nu * hv_factor(k2, ph.p);
+  if (ph.damping) {
+    double ls = (fl & FLAG_LSDAMP) ? -ph.mu : 0.0;
+    double dh = dc ? 0.0 : -ph.nu_d * hv_factor(k2, ph.p_d);
+    l = l + (ls + dh);
+  }
+  return l;
+}
+
+// Linear part of the explicit RHS, in the reference's order:
+// (adv + hv*y) + s*F + (ls*y + dhv*y)  (solver.py:170-178, damping.py:75-92).
+__device__ __forceinline__ cd explicit_rhs(cd adv, cd y, double k2, bool dc, unsigned fl,
+                                           const Phys& ph, bool forced, cd sF) {
+  double hv = (dc && ph.hv_dc_zero) ? 0.0 : -ph.nu * hv_factor(k2, ph.p);
+  cd k = cadd(adv, cscale(y, hv));
+  if (forced) k = cadd(k, sF);
+  if (ph.damping) {
+    cd ls = (fl & FLAG_LSDAMP) ? cscale(y, -ph.mu) : cmk(0.0, 0.0);
+    double dh = dc ? 0.0 : -ph.nu_d * hv_factor(k2, ph.p_d);
+    k = cadd(k, cadd(ls, cscale(y, dh)));
+  }
+  return k;
+}
+
+// Hermitian (PF) and anti-Hermitian (QF) parts of the unscaled ring forcing
+// F = amp * exp(i phi) at mode (i, jy) (ring_forcing.py:138-150: phases drawn
+// over the full plane, DC and the Nyquist corners made real).
+__device__ __forceinline__ void ring_parts(uint64_t key, int i, int jy, int N, double amp, cd& PF,
+                                           cd& QF) {
+  const int mi = (N - i) & (N - 1), mj = (N - jy) & (N - 1);
+  double s1, c1, s2, c2;
+  sincos(phase_of(key, (uint64_t)i * (uint64_t)N + (uint64_t)jy), &s1, &c1);
+  sincos(phase_of(key, (uint64_t)mi * (uint64_t)N + (uint64_t)mj), &s2, &c2);
+  cd f1 = cmk(amp * c1, amp * s1);
+  cd f2 = cmk(amp * c2, amp * s2);
+  const bool selfc = (i == 0 || i == N / 2) && (jy == 0 || jy == N / 2);
+  if (selfc) {
+    f1.y = 0.0;
+    f2 = f1;
+  }
+  PF = cscale(cadd(f1, cconj(f2)), 0.5);
+  QF = cscale(csub(f1, cconj(f2)), 0.5);
+}
+
+__device__ __forceinline__ uint64_t ctrl_key(const Ctrl* c, int64_t step) {
+  return c->key_mode ? step_key(c->seed, step) : c->key;
+}
+
+// Packed row 0 helpers: Z = X0 + i XN with X0, XN Hermitian along kx.
+__device__ __forceinline__ cd unpack0(cd z, cd zp) { return cscale(cadd(z, cconj(zp)), 0.5); }
+__device__ __forceinline__ cd unpackN(cd z, cd zp) {
+  return cscale(cmul_mi(csub(z, cconj(zp))), 0.5);
+}
+__device__ __forceinline__ cd pack(cd x0, cd xn) { return cadd(x0, cmul_i(xn)); }
+
+// Per-mode epilogue state.
+struct ModeIn {
+  cd J;    // forward-transformed Jacobian (not yet dealiased / negated)
+  cd th;   // theta_hat_n (Hermitian part)
+  cd acc;  // RK accumulator
+  cd ys;   // stored stage input
+};
+struct ModeOut {
+  cd th, acc, ys, y;  // y = input of the next stage
+};
+
+// Stage e epilogue at one mode.  Explicit scheme follows
+// core/time_integrator.py:30-36 (and core/solver_optimized.py:62 for the
+// multistep combine); the IF scheme is Lawson RK4 with the diagonal linear
+// operator integrated exactly.
+template <bool SCALAR>
+__device__ __forceinline__ ModeOut mode_epilogue(int e, const ModeIn& in, double k2, bool dc,
+                                                 unsigned fl, const Phys& ph, double dt,
+                                                 bool forced, cd sF) {
+  ModeOut o;
+  o.th = in.th;
+  o.acc = in.acc;
+  o.ys = in.ys;
+  const double hdt = 0.5 * dt;
+  cd adv = (fl & FLAG_DEALIAS) ? cmk(-in.J.x, -in.J.y) : cmk(0.0, 0.0);
+  if (ph.scheme == 0) {
+    cd y;
+    if (e == 1)
+      y = in.th;
+    else if (e == 2)
+      y = cadd(in.th, cscale(in.acc, hdt));
+    else
+      y = in.ys;
+    cd k;
+    if (SCALAR) {
+      k = cadd(adv, cscale(y, -ph.kappa * k2));
+    } else {
+      k = explicit_rhs(adv, y, k2, dc, fl, ph, forced, sF);
+    }
+    if (e == 1) {
+      o.acc = k;
+      o.y = cadd(in.th, cscale(k, hdt));
+    } else if (e == 2) {
+      o.acc = cadd(in.acc, cscale(k, 2.0));
+      o.ys = cadd(in.th, cscale(k, hdt));
+      o.y = o.ys;
+    } else if (e == 3) {
+      o.acc = cadd(in.acc, cscale(k, 2.0));
+      o.ys = cadd(in.th, cscale(k, dt));
+      o.y = o.ys;
+    } else {
+      cd s = cadd(in.acc, k);
+      cd d;
+      if (ph.combine == 0) {
+        cd t = cscale(s, dt);
+        d = cmk(t.x / 6.0, t.y / 6.0);
+      } else {
+        d = cscale(s, dt / 6.0);
+      }
+      o.th = cadd(in.th, d);
+      o.y = o.th;
+    }
+  } else {
+    double L = SCALAR ? -ph.kappa * k2 : lin_coeff(k2, dc, fl, ph);
+    double e1 = exp(L * hdt), e2 = exp(L * dt);
+    cd k = adv;
+    if (!SCALAR && forced) k = cadd(k, sF);
+    if (e == 1) {
+      o.acc = cscale(k, e2);
+      o.y = cscale(cadd(in.th, cscale(k, hdt)), e1);
+    } else if (e == 2) {
+      o.acc = cadd(in.acc, cscale(cscale(k, e1), 2.0));
+      o.y = cadd(cscale(in.th, e1), cscale(k, hdt));
+    } else if (e == 3) {
+      cd ek = cscale(k, e1);
+      o.acc = cadd(in.acc, cscale(ek, 2.0));
+      o.y = cadd(cscale(in.th, e2), cscale(ek, dt));
+    } else {
+      o.th = cadd(cscale(in.th, e2), cscale(cadd(in.acc, k), dt / 6.0));
+      o.y = o.th;
+    }
+  }
+  return o;
+}
+
+// Prologue multipliers (Hermitianised, core/operators.py:27-28,90-91 with
+// the .real of ifft2 folded in: i kx vanishes on the kx-Nyquist row and i ky
+// on the ky-Nyquist column), including the 1/N^2 of the inverse transform.
+template <int JOB>
+__device__ __forceinline__ void prologue_fields(cd y, double kx, double ky, double k2, bool kxn,
+                                                bool kyn, const Phys& ph, cd (&f)[4]) {
+  const double s = ph.inv_n2;
+  const double cx = kxn ? 0.0 : kx * s;
+  const double cy = kyn ? 0.0 : ky * s;
+  if (JOB == CK_THETA) {
+    const double K = inv_frac(k2, ph);
+    f[0] = cmul_i(cscale(y, cx));          // theta_x
+    f[1] = cmul_i(cscale(y, cy));          // theta_y
+    f[2] = cmul_mi(cscale(y, cy * K));     // u = -d_y psi
+    f[3] = cmul_i(cscale(y, cx * K));      // v =  d_x psi
+  } else if (JOB == CK_SCALAR) {
+    f[0] = cmul_i(cscale(y, cx));
+    f[1] = cmul_i(cscale(y, cy));
+    f[2] = f[3] = cmk(0.0, 0.0);
+  } else {  // CK_VEL
+    const double K = inv_frac(k2, ph);
+    f[0] = cmul_mi(cscale(y, cy * K));
+    f[1] = cmul_i(cscale(y, cx * K));
+    f[2] = f[3] = cmk(0.0, 0.0);
+  }
+}
+
+template <int JOB>
+struct NFields {
+  static constexpr int value = JOB == CK_THETA ? 4 : 2;
+};
+
+__device__ __forceinline__ unsigned mode_flags(const uint8_t* flags, int N, int i, int jy) {
+  if (jy == N / 2) return (unsigned)(flags[i] >> 4);
+  return (unsigned)(flags[(size_t)jy * N + i] & 15u);
+}
+
+// ------------------------------------------------------------ column kernel
+template <int N, int JOB>
+__global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
+  using S = CKShape<N>;
+  constexpr int Q = S::Q;
+  constexpr int PADL = S::PADL;
+  constexpr int NT = S::NT;
+  __shared__ cd lds[S::RPW * PADL];
+  __shared__ double red[NT];
+
+  const int tid = threadIdx.x;
+  const int rl = tid / Q;
+  const int t = tid - rl * Q;
+  const int j = blockIdx.x * S::RPW + rl;
+  const size_t fo = (size_t)blockIdx.y * (size_t)a.field_stride;
+  const size_t ro = fo + (size_t)j * N;
+  cd* buf = lds + rl * PADL;
+  const Phys& ph = a.ph;
+  const double* k1d = a.k1d;
+  const cd* __restrict__ tw = a.tw;
+  const bool row0 = (j == 0);
+  const bool blk0 = (blockIdx.x == 0);  // block-uniform: row 0 lives in block 0
+
+  cd v[8];
+
+  // ---------------------------------------------------- API transforms
+  if (JOB == CK_FWD || JOB == CK_INV) {
+    const cd* src = (JOB == CK_FWD) ? a.J : a.th;
+    cd* dst = (JOB == CK_FWD) ? a.th : a.F[0];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = src[ro + t + m * Q];
+    if (JOB == CK_INV) {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) v[m] = cscale(v[m], ph.inv_n2);
+      fft_lanes<N, +1>(v, buf, t, tw);
+    } else {
+      fft_lanes<N, -1>(v, buf, t, tw);
+    }
+#pragma unroll
+    for (int m = 0; m < 8; ++m) dst[ro + t + m * Q] = v[m];
+    return;
+  }
+
+  const Ctrl* ctrl = a.ctrl;
+  const int64_t step = ctrl->step;
+  const bool forced = (JOB == CK_THETA) && ph.forcing;
+
+  // y[m] (packed in row 0) is the prologue state.
+  cd y[8];
+  const int e = a.e;
+
+  if (e > 0) {
+    // Issue every load up front; the forward FFT covers their latency.
+    cd thv[8], accv[8], ysv[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = a.J[ro + t + m * Q];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) thv[m] = a.th[ro + t + m * Q];
+    const bool need_acc = (e >= 2);
+    const bool need_ys = (ph.scheme == 0) && (e >= 3);
+    if (need_acc) {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) accv[m] = a.acc[ro + t + m * Q];
+    } else {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) accv[m] = cmk(0.0, 0.0);
+    }
+    if (need_ys) {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) ysv[m] = a.ys[ro + t + m * Q];
+    } else {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) ysv[m] = cmk(0.0, 0.0);
+    }
+
+    fft_lanes<N, -1>(v, buf, t, tw);
+
+    double sc = 1.0;
+    uint64_t key = 0;
+    if (forced) {
+      sc = ctrl->scale[e];
+      key = ctrl_key(ctrl, step);
+    }
+
+    if (blk0) {
+      // Partner values X(-i) of packed row 0: the Jacobian through LDS, the
+      // state from global.  Barriers are taken by the whole of block 0.
+      __syncthreads();
+      if (row0) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) buf[lpad(t + m * Q)] = v[m];
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int i = t + m * Q;
+      const double kx = k1d[i];
+      ModeIn in;
+      in.J = v[m];
+      in.th = thv[m];
+      in.acc = accv[m];
+      in.ys = ysv[m];
+      if (!row0) {
+        const int jy = j;
+        const double ky = k1d[jy];
+        const double k2 = kx * kx + ky * ky;
+        const unsigned fl = mode_flags(a.flags, N, i, jy);
+        cd sF = cmk(0.0, 0.0);
+        if (forced && (fl & FLAG_RING)) {
+          cd PF, QF;
+          ring_parts(key, i, jy, N, ph.amp, PF, QF);
+          sF = cscale(PF, sc);
+        }
+        ModeOut o = mode_epilogue<JOB == CK_SCALAR>(e, in, k2, false, fl, ph, a.dt, forced, sF);
+        if (e == 1) a.acc[ro + i] = o.acc;
+        if (e == 2 || e == 3) {
+          a.acc[ro + i] = o.acc;
+          if (ph.scheme == 0) a.ys[ro + i] = o.ys;
+        }
+        if (e == 4) a.th[ro + i] = o.th;
+        y[m] = o.y;
+      } else {
+        const int ip = (N - i) & (N - 1);
+        const cd Jp = buf[lpad(ip)];
+        const cd thp = a.th[ro + ip];
+        const cd accp = need_acc ? a.acc[ro + ip] : cmk(0.0, 0.0);
+        const cd ysp = need_ys ? a.ys[ro + ip] : cmk(0.0, 0.0);
+        ModeOut oo[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int jy = h ? N / 2 : 0;
+          const double ky = k1d[jy];
+          const double k2 = kx * kx + ky * ky;
+          const unsigned fl = mode_flags(a.flags, N, i, jy);
+          ModeIn u;
+          u.J = h ? unpackN(in.J, Jp) : unpack0(in.J, Jp);
+          u.th = h ? unpackN(in.th, thp) : unpack0(in.th, thp);
+          u.acc = h ? unpackN(in.acc, accp) : unpack0(in.acc, accp);
+          u.ys = h ? unpackN(in.ys, ysp) : unpack0(in.ys, ysp);
+          cd sF = cmk(0.0, 0.0);
+          if (forced && (fl & FLAG_RING)) {
+            cd PF, QF;
+            ring_parts(key, i, jy, N, ph.amp, PF, QF);
+            sF = cscale(PF, sc);
+          }
+          const bool dc = (i == 0) && (jy == 0);
+          oo[h] = mode_epilogue<JOB == CK_SCALAR>(e, u, k2, dc, fl, ph, a.dt, forced, sF);
+        }
+        if (e == 1) a.acc[ro + i] = pack(oo[0].acc, oo[1].acc);
+        if (e == 2 || e == 3) {
+          a.acc[ro + i] = pack(oo[0].acc, oo[1].acc);
+          if (ph.scheme == 0) a.ys[ro + i] = pack(oo[0].ys, oo[1].ys);
+        }
+        if (e == 4) {
+          a.th[ro + i] = pack(oo[0].th, oo[1].th);
+        }
+        y[m] = pack(oo[0].y, oo[1].y);
+      }
+    }
+    if (e == 4 && JOB == CK_THETA) {
+      bool bad = false;
+#pragma unroll
+      for (int m = 0; m < 8; ++m) bad |= !(isfinite(y[m].x) && isfinite(y[m].y));
+      if (bad) a.ctrl->nonfinite = 1.0;
+    }
+
+    // Anti-Hermitian ring side state (only the final stage, theta only).
+    if (JOB == CK_THETA && e == 4 && forced && a.n_a > 0) {
+      const uint64_t k4 = key;
+      const double s1 = ctrl->scale[1], s2 = ctrl->scale[2], s3 = ctrl->scale[3], s4 = ctrl->scale[4];
+      for (int ent = blockIdx.x + gridDim.x * tid; ent < a.n_a; ent += gridDim.x * NT) {
+        const int2 md = a.a_modes[ent];
+        const int ii = md.x, jy = md.y;
+        const double kx = k1d[ii], ky = k1d[jy];
+        const double k2 = kx * kx + ky * ky;
+        const unsigned fl = mode_flags(a.flags, N, ii, jy);
+        const bool dc = (ii == 0) && (jy == 0);
+        cd PF, QF;
+        ring_parts(k4, ii, jy, N, ph.amp, PF, QF);
+        const cd A = a.a_val[ent];
+        const cd zero = cmk(0.0, 0.0);
+        cd An;
+        if (ph.scheme == 0) {
+          const double hdt = 0.5 * a.dt;
+          cd k1 = explicit_rhs(zero, A, k2, dc, fl, ph, true, cscale(QF, s1));
+          cd acc = k1;
+          cd y2 = cadd(A, cscale(k1, hdt));
+          cd k2v = explicit_rhs(zero, y2, k2, dc, fl, ph, true, cscale(QF, s2));
+          acc = cadd(acc, cscale(k2v, 2.0));
+          cd y3 = cadd(A, cscale(k2v, hdt));
+          cd k3 = explicit_rhs(zero, y3, k2, dc, fl, ph, true, cscale(QF, s3));
+          acc = cadd(acc, cscale(k3, 2.0));
+          cd y4 = cadd(A, cscale(k3, a.dt));
+          cd k4v = explicit_rhs(zero, y4, k2, dc, fl, ph, true, cscale(QF, s4));
+          cd s = cadd(acc, k4v);
+          cd d;
+          if (ph.combine == 0) {
+            cd tt = cscale(s, a.dt);
+            d = cmk(tt.x / 6.0, tt.y / 6.0);
+          } else {
+            d = cscale(s, a.dt / 6.0);
+          }
+          An = cadd(A, d);
+        } else {
+          const double L = lin_coeff(k2, dc, fl, ph);
+          const double e1 = exp(L * 0.5 * a.dt), e2 = exp(L * a.dt);
+          cd k1 = cscale(QF, s1), k2v = cscale(QF, s2), k3 = cscale(QF, s3), k4v = cscale(QF, s4);
+          cd acc = cscale(k1, e2);
+          acc = cadd(acc, cscale(cscale(k2v, e1), 2.0));
+          acc = cadd(acc, cscale(cscale(k3, e1), 2.0));
+          An = cadd(cscale(A, e2), cscale(cadd(acc, k4v), a.dt / 6.0));
+        }
+        a.a_val[ent] = An;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) y[m] = a.th[ro + t + m * Q];
+  }
+
+  // -------------------------------------------------------------- prologue
+  if (!a.prologue) return;
+
+  // <theta f> partials of the prologue state (ring_forcing.py:153-155 by
+  // Parseval over the ring modes).
+  if (forced && a.rate_slot > 0) {
+    const uint64_t key = ctrl_key(ctrl, a.rate_next_key ? step + 1 : step);
+    double accr = 0.0;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int i = t + m * Q;
+      if (!row0) {
+        const unsigned fl = mode_flags(a.flags, N, i, j);
+        if (fl & FLAG_RING) {
+          cd PF, QF;
+          ring_parts(key, i, j, N, ph.amp, PF, QF);
+          accr += 2.0 * (y[m].x * PF.x + y[m].y * PF.y);
+        }
+      } else {
+        // needs y(-i): stage through LDS below
+      }
+    }
+    if (blk0) {
+      __syncthreads();
+      if (row0) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) buf[lpad(t + m * Q)] = y[m];
+      }
+      __syncthreads();
+    }
+    if (row0) {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const int i = t + m * Q;
+        const int ip = (N - i) & (N - 1);
+        const cd yp = buf[lpad(ip)];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int jy = h ? N / 2 : 0;
+          const unsigned fl = mode_flags(a.flags, N, i, jy);
+          if (fl & FLAG_RING) {
+            cd PF, QF;
+            ring_parts(key, i, jy, N, ph.amp, PF, QF);
+            cd yy = h ? unpackN(y[m], yp) : unpack0(y[m], yp);
+            accr += yy.x * PF.x + yy.y * PF.y;
+          }
+        }
+      }
+    }
+    red[tid] = accr;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+      if (tid < s) red[tid] += red[tid + s];
+      __syncthreads();
+    }
+    if (tid == 0) a.part[(size_t)a.rate_slot * a.part_stride + blockIdx.x] = red[0];
+  }
+
+  // Multipliers and inverse x-FFTs.
+  cd yp_row0[8];
+  if (blk0) {
+    __syncthreads();
+    if (row0) {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) buf[lpad(t + m * Q)] = y[m];
+    }
+    __syncthreads();
+    if (row0) {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) yp_row0[m] = buf[lpad((N - (t + m * Q)) & (N - 1))];
+    }
+  }
+  constexpr int NF = NFields<JOB>::value;
+#pragma unroll
+  for (int q = 0; q < NF; ++q) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int i = t + m * Q;
+      const double kx = k1d[i];
+      const bool kxn = (i == N / 2);
+      cd f[4];
+      if (!row0) {
+        const double ky = k1d[j];
+        prologue_fields<JOB>(y[m], kx, ky, kx * kx + ky * ky, kxn, false, ph, f);
+        v[m] = f[q];
+      } else {
+        cd f0[4], fN[4];
+        const double kyN = k1d[N / 2];
+        prologue_fields<JOB>(unpack0(y[m], yp_row0[m]), kx, 0.0, kx * kx, kxn, false, ph, f0);
+        prologue_fields<JOB>(unpackN(y[m], yp_row0[m]), kx, kyN, kx * kx + kyN * kyN, kxn, true,
+                             ph, fN);
+        v[m] = pack(f0[q], fN[q]);
+      }
+    }
+    __syncthreads();
+    fft_lanes<N, +1>(v, buf, t, tw);
+    cd* dst = a.F[q] + fo + (size_t)j * N;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) dst[t + m * Q] = v[m];
+  }
+}
+
+// --------------------------------------------------------------- row kernel
+template <int N>
+__device__ __forceinline__ int rk_tile(int b) {
+  constexpr int T = N / RKShape<N>::WX;
+  if (T % 8 != 0) return b;
+  // Blocks are dispatched round-robin over the 8 XCDs: give each XCD a
+  // contiguous run of x-tiles so that tiles sharing 128-byte lines of the
+  // ky-major fields meet in the same L2.
+  return (b & 7) * (T / 8) + (b >> 3);
+}
+
+// Load one mixed field's tile into LDS (coalesced along x), then pre-twist
+// for the half-length complex inverse: lane element k gets
+// Z'(k) = (X(k) + conj X(M-k)) + i e^{+2 pi i k/N} (X(k) - conj X(M-k)).
+template <int N>
+__device__ __forceinline__ void rk_load_c2r(const cd* __restrict__ src, int x0, cd* lds, int tid,
+                                            int b, int t, const cd* __restrict__ tw, cd (&v)[8]) {
+  using S = RKShape<N>;
+  constexpr int M = S::M, WX = S::WX, NT = S::NT, PADM = S::PADM, Q = S::Q;
+  cd tmp[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int l = tid + r * NT;
+    const int jr = l / WX, bb = l - jr * WX;
+    tmp[r] = src[(size_t)jr * N + x0 + bb];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int l = tid + r * NT;
+    const int jr = l / WX, bb = l - jr * WX;
+    lds[bb * PADM + lpad(jr)] = tmp[r];
+  }
+  __syncthreads();
+  const cd* buf = lds + b * PADM;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int k = t + m * Q;
+    const cd X = buf[lpad(k)];
+    if (k == 0) {
+      v[m] = cmk(X.x + X.y, X.x - X.y);
+    } else {
+      const cd Xp = cconj(buf[lpad(M - k)]);
+      const cd w = cconj(tw[N + k]);
+      v[m] = cadd(cadd(X, Xp), cmul_i(cmul(w, csub(X, Xp))));
+    }
+  }
+  __syncthreads();
+}
+
+// Forward half-length FFT result in v -> post-twist -> store X(k), packed
+// slot 0 = X(0) + i X(M), at dst[k*N + x0 + b] (coalesced along x).
+template <int N>
+__device__ __forceinline__ void rk_store_r2c(cd (&v)[8], cd* dst, int x0, cd* lds, int tid, int b,
+                                             int t, const cd* __restrict__ tw) {
+  using S = RKShape<N>;
+  constexpr int M = S::M, WX = S::WX, NT = S::NT, PADM = S::PADM, Q = S::Q;
+  cd* buf = lds + b * PADM;
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < 8; ++m) buf[lpad(t + m * Q)] = v[m];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int l = tid + r * NT;
+    const int jr = l / WX, bb = l - jr * WX;
+    const cd* bf = lds + bb * PADM;
+    const cd Z = bf[lpad(jr)];
+    cd X;
+    if (jr == 0) {
+      X = cmk(Z.x + Z.y, Z.x - Z.y);
+    } else {
+      const cd Zp = cconj(bf[lpad(M - jr)]);
+      const cd E = cscale(cadd(Z, Zp), 0.5);
+      const cd O = cscale(cmul_mi(csub(Z, Zp)), 0.5);
+      X = cadd(E, cmul(tw[N + jr], O));
+    }
+    dst[(size_t)jr * N + x0 + bb] = X;
+  }
+}
+
+template <int N, int JOB>
+__global__ void __launch_bounds__(RKShape<N>::NT) rk_kernel(RKArgs a) {
+  using S = RKShape<N>;
+  constexpr int M = S::M, Q = S::Q, WX = S::WX, NT = S::NT, PADM = S::PADM;
+  __shared__ cd lds[WX * PADM];
+  __shared__ double red[NT];
+  const int tid = threadIdx.x;
+  const int b = tid / Q;
+  const int t = tid - b * Q;
+  const int x0 = rk_tile<N>(blockIdx.x) * WX;
+  const int x = x0 + b;
+  const size_t fo = (size_t)blockIdx.y * (size_t)a.field_stride;
+  const cd* __restrict__ tw = a.tw;
+  cd* buf = lds + b * PADM;
+
+  // Reduce the <theta f> partials of stage red_slot (block 0 only; every
+  // other block is independent of it).
+  if (a.red_slot > 0 && blockIdx.x == 0 && blockIdx.y == 0) {
+    double s = 0.0;
+    const double* p = a.part + (size_t)a.red_slot * a.part_stride;
+    for (int i = tid; i < a.nblk_ck; i += NT) s += p[i];
+    red[tid] = s;
+    __syncthreads();
+    for (int h = NT / 2; h > 0; h >>= 1) {
+      if (tid < h) red[tid] += red[tid + h];
+      __syncthreads();
+    }
+    if (tid == 0) {
+      const double inv = 1.0 / ((double)N * (double)N);
+      const double rate = (red[0] * inv) * inv;
+      a.ctrl->rate[a.red_slot] = rate;
+      a.ctrl->scale[a.red_slot] = fabs(rate) > 1e-10 ? a.eps / rate : 1.0;
+      if (a.inc_step) a.ctrl->step = a.ctrl->step + 1;
+    }
+  } else if (a.inc_step && blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) {
+    a.ctrl->step = a.ctrl->step + 1;
+  }
+
+  cd v[8];
+  if (JOB == RK_R2C) {
+    const cd* src = reinterpret_cast<const cd*>(a.phys_in + fo + (size_t)x * N);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = src[t + m * Q];
+    fft_lanes<M, -1>(v, buf, t, tw);
+    rk_store_r2c<N>(v, a.Jout + fo, x0, lds, tid, b, t, tw);
+    return;
+  }
+  if (JOB == RK_C2R || JOB == RK_VEL) {
+    const int nf = (JOB == RK_C2R) ? 1 : 2;
+    for (int q = 0; q < nf; ++q) {
+      rk_load_c2r<N>(a.F[q] + fo, x0, lds, tid, b, t, tw, v);
+      fft_lanes<M, +1>(v, buf, t, tw);
+      double* dstp = (JOB == RK_C2R) ? a.phys_out : (q == 0 ? a.uout : a.vout);
+      cd* dst = reinterpret_cast<cd*>(dstp + fo + (size_t)x * N);
+#pragma unroll
+      for (int m = 0; m < 8; ++m) dst[t + m * Q] = v[m];
+    }
+    return;
+  }
+
+  // J = u * theta_x + v * theta_y on packed real pairs z(k) = (r(2k), r(2k+1)).
+  cd jz[8], hold[8];
+  if (JOB == RK_THETA) {
+    // theta_x
+    rk_load_c2r<N>(a.F[0] + fo, x0, lds, tid, b, t, tw, v);
+    fft_lanes<M, +1>(v, buf, t, tw);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) hold[m] = v[m];
+    // u
+    rk_load_c2r<N>(a.F[2] + fo, x0, lds, tid, b, t, tw, v);
+    fft_lanes<M, +1>(v, buf, t, tw);
+    if (a.uout) {
+      cd* dst = reinterpret_cast<cd*>(a.uout + (size_t)x * N);
+#pragma unroll
+      for (int m = 0; m < 8; ++m) dst[t + m * Q] = v[m];
+    }
+#pragma unroll
+    for (int m = 0; m < 8; ++m) jz[m] = cmk(v[m].x * hold[m].x, v[m].y * hold[m].y);
+    // theta_y
+    rk_load_c2r<N>(a.F[1] + fo, x0, lds, tid, b, t, tw, v);
+    fft_lanes<M, +1>(v, buf, t, tw);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) hold[m] = v[m];
+    // v
+    rk_load_c2r<N>(a.F[3] + fo, x0, lds, tid, b, t, tw, v);
+    fft_lanes<M, +1>(v, buf, t, tw);
+    if (a.vout) {
+      cd* dst = reinterpret_cast<cd*>(a.vout + (size_t)x * N);
+#pragma unroll
+      for (int m = 0; m < 8; ++m) dst[t + m * Q] = v[m];
+    }
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+      jz[m] = cmk(jz[m].x + v[m].x * hold[m].x, jz[m].y + v[m].y * hold[m].y);
+  } else {  // RK_SCALAR: frozen physical u, v
+    const cd* up = reinterpret_cast<const cd*>(a.uphys + (size_t)x * N);
+    const cd* vp = reinterpret_cast<const cd*>(a.vphys + (size_t)x * N);
+    cd uu[8], vv[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      uu[m] = up[t + m * Q];
+      vv[m] = vp[t + m * Q];
+    }
+    rk_load_c2r<N>(a.F[0] + fo, x0, lds, tid, b, t, tw, v);
+    fft_lanes<M, +1>(v, buf, t, tw);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) jz[m] = cmk(uu[m].x * v[m].x, uu[m].y * v[m].y);
+    rk_load_c2r<N>(a.F[1] + fo, x0, lds, tid, b, t, tw, v);
+    fft_lanes<M, +1>(v, buf, t, tw);
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+      jz[m] = cmk(jz[m].x + vv[m].x * v[m].x, jz[m].y + vv[m].y * v[m].y);
+  }
+#pragma unroll
+  for (int m = 0; m < 8; ++m) v[m] = jz[m];
+  __syncthreads();
+  fft_lanes<M, -1>(v, buf, t, tw);
+  rk_store_r2c<N>(v, a.Jout + fo, x0, lds, tid, b, t, tw);
+}
+
+#if defined(GSQG_INSTANTIATE_COMMON)
+// ------------------------------------------------------- layout converters
+// full (N, N) [kx][ky] complex -> half-packed ky-major P(X), rows 1..N/2-1.
+// Also accumulates sum |Q(X)|^2 (weight 2: the mirror mode is not visited)
+// per block into norms[blockIdx].
+template <int TILE>
+__global__ void full_to_half_kernel(const cd* __restrict__ X, cd* __restrict__ H, int N,
+                                    double* norms) {
+  __shared__ cd a[TILE][TILE + 1];
+  __shared__ cd mrr[TILE][TILE + 1];
+  __shared__ double red[256];
+  const int i0 = blockIdx.x * TILE;  // kx
+  const int j0 = blockIdx.y * TILE;  // ky (only 0..N/2)
+  const int tx = threadIdx.x % TILE, ty = threadIdx.x / TILE;
+  const int rows_per = 256 / TILE;
+  for (int r = ty; r < TILE; r += rows_per) {
+    const int i = i0 + r;
+    const int jy = j0 + tx;
+    if (jy < N / 2 && jy > 0) {
+      a[r][tx] = X[(size_t)i * N + jy];
+      const int mi = (N - i) & (N - 1), mj = N - jy;
+      mrr[r][tx] = X[(size_t)mi * N + mj];
+    }
+  }
+  __syncthreads();
+  double nrm = 0.0;
+  for (int r = ty; r < TILE; r += rows_per) {
+    const int jy = j0 + r;
+    const int i = i0 + tx;
+    if (jy < N / 2 && jy > 0) {
+      const cd z = a[tx][r], zm = cconj(mrr[tx][r]);
+      H[(size_t)jy * N + i] = cscale(cadd(z, zm), 0.5);
+      const cd q = cscale(csub(z, zm), 0.5);
+      nrm += 2.0 * (q.x * q.x + q.y * q.y);
+    }
+  }
+  red[threadIdx.x] = nrm;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && norms) norms[blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+
+// Row 0 of the half-packed layout (ky = 0 and ky = N/2 columns).
+__global__ void full_to_half_row0_kernel(const cd* __restrict__ X, cd* __restrict__ H, int N,
+                                         double* norm0) {
+  __shared__ double red[256];
+  double nrm = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+    const int mi = (N - i) & (N - 1);
+    const cd a0 = X[(size_t)i * N], b0 = cconj(X[(size_t)mi * N]);
+    const cd aN = X[(size_t)i * N + N / 2], bN = cconj(X[(size_t)mi * N + N / 2]);
+    const cd p0 = cscale(cadd(a0, b0), 0.5), pN = cscale(cadd(aN, bN), 0.5);
+    H[i] = pack(p0, pN);
+    const cd q0 = cscale(csub(a0, b0), 0.5), qN = cscale(csub(aN, bN), 0.5);
+    nrm += q0.x * q0.x + q0.y * q0.y + qN.x * qN.x + qN.y * qN.y;
+  }
+  red[threadIdx.x] = nrm;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && norm0) norm0[blockIdx.x] = red[0];
+}
+
+// half-packed ky-major -> full (N, N) [kx][ky] with Hermitian extension.
+template <int TILE>
+__global__ void half_to_full_kernel(const cd* __restrict__ H, cd* __restrict__ X, int N) {
+  __shared__ cd a[TILE][TILE + 1];
+  const int i0 = blockIdx.x * TILE;  // kx (output rows)
+  const int c0 = blockIdx.y * TILE;  // ky (output columns)
+  const int tx = threadIdx.x % TILE, ty = threadIdx.x / TILE;
+  const int rows_per = 256 / TILE;
+  // Gather: for output column jy = c0 + r, read along kx (contiguous in H).
+  for (int r = ty; r < TILE; r += rows_per) {
+    const int jy = c0 + r;
+    const int i = i0 + tx;
+    cd val = cmk(0.0, 0.0);
+    if (jy > 0 && jy < N / 2) {
+      val = H[(size_t)jy * N + i];
+    } else if (jy > N / 2) {
+      val = cconj(H[(size_t)(N - jy) * N + ((N - i) & (N - 1))]);
+    } else {
+      const cd z = H[i], zp = H[(N - i) & (N - 1)];
+      val = (jy == 0) ? unpack0(z, zp) : unpackN(z, zp);
+    }
+    a[tx][r] = val;
+  }
+  __syncthreads();
+  for (int r = ty; r < TILE; r += rows_per) {
+    const int i = i0 + r;
+    X[(size_t)i * N + c0 + tx] = a[r][tx];
+  }
+}
+
+// Anti-Hermitian side state <-> full spectra.
+__global__ void a_gather_kernel(const cd* __restrict__ X, const int2* modes, cd* aval, int n,
+                                int N, double* norm) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const int i = modes[e].x, jy = modes[e].y;
+    const int mi = (N - i) & (N - 1), mj = (N - jy) & (N - 1);
+    const cd z = X[(size_t)i * N + jy], zm = cconj(X[(size_t)mi * N + mj]);
+    const cd q = cscale(csub(z, zm), 0.5);
+    aval[e] = q;
+    const double w = (jy == 0 || jy == N / 2) ? 1.0 : 2.0;
+    s += w * (q.x * q.x + q.y * q.y);
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && norm) norm[blockIdx.x] = red[0];
+}
+
+__global__ void a_scatter_kernel(cd* X, const int2* modes, const cd* aval, int n, int N) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const int i = modes[e].x, jy = modes[e].y;
+    const cd A = aval[e];
+    const size_t o = (size_t)i * N + jy;
+    X[o] = cadd(X[o], A);
+    if (jy != 0 && jy != N / 2) {
+      const int mi = (N - i) & (N - 1), mj = N - jy;
+      const size_t om = (size_t)mi * N + mj;
+      X[om] = csub(X[om], cconj(A));
+    }
+  }
+}
+
+// Anti-Hermitian part of compute_rhs at the ring side-state modes.
+__global__ void a_rhs_kernel(Phys ph, const double* k1d, const uint8_t* flags, const Ctrl* ctrl,
+                             const int2* modes, const cd* aval, cd* out, int n, int N) {
+  const uint64_t key = ctrl_key(ctrl, ctrl->step);
+  const double s1 = ctrl->scale[1];
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const int i = modes[e].x, jy = modes[e].y;
+    const double kx = k1d[i], ky = k1d[jy];
+    const double k2 = kx * kx + ky * ky;
+    const unsigned fl = mode_flags(flags, N, i, jy);
+    cd PF, QF;
+    ring_parts(key, i, jy, N, ph.amp, PF, QF);
+    const cd A = aval[e];
+    out[e] = explicit_rhs(cmk(0.0, 0.0), A, k2, (i == 0 && jy == 0), fl, ph, ph.forcing != 0,
+                          cscale(QF, s1));
+  }
+}
+
+__global__ void set_ctrl_kernel(Ctrl* c, int64_t step, uint64_t key, uint64_t seed, int key_mode) {
+  c->step = step;
+  c->key = key;
+  c->seed = seed;
+  c->key_mode = key_mode;
+}
+
+#endif  // GSQG_INSTANTIATE_COMMON
+
+// ------------------------------------------------------------- launchers
+// The build compiles this file once per grid size (GSQG_INSTANTIATE_N) and
+// once for the size-independent parts (GSQG_INSTANTIATE_COMMON).
+template <int N, int JOB>
+static hipError_t launch_ck_n(const CKArgs& a, int nfields_y, hipStream_t s) {
+  using S = CKShape<N>;
+  dim3 grid((N / 2) / S::RPW, nfields_y);
+  hipLaunchKernelGGL((ck_kernel<N, JOB>), grid, dim3(S::NT), 0, s, a);
+  return hipGetLastError();
+}
+
+template <int N, int JOB>
+static hipError_t launch_rk_n(const RKArgs& a, int nfields_y, hipStream_t s) {
+  using S = RKShape<N>;
+  dim3 grid(N / S::WX, nfields_y);
+  hipLaunchKernelGGL((rk_kernel<N, JOB>), grid, dim3(S::NT), 0, s, a);
+  return hipGetLastError();
+}
+
+template <int N>
+hipError_t ck_by_job(int job, const CKArgs& a, int ny, hipStream_t s) {
+  switch (job) {
+    case CK_THETA: return launch_ck_n<N, CK_THETA>(a, ny, s);
+    case CK_SCALAR: return launch_ck_n<N, CK_SCALAR>(a, ny, s);
+    case CK_VEL: return launch_ck_n<N, CK_VEL>(a, ny, s);
+    case CK_FWD: return launch_ck_n<N, CK_FWD>(a, ny, s);
+    case CK_INV: return launch_ck_n<N, CK_INV>(a, ny, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <int N>
+hipError_t rk_by_job(int job, const RKArgs& a, int ny, hipStream_t s) {
+  switch (job) {
+    case RK_THETA: return launch_rk_n<N, RK_THETA>(a, ny, s);
+    case RK_SCALAR: return launch_rk_n<N, RK_SCALAR>(a, ny, s);
+    case RK_VEL: return launch_rk_n<N, RK_VEL>(a, ny, s);
+    case RK_R2C: return launch_rk_n<N, RK_R2C>(a, ny, s);
+    case RK_C2R: return launch_rk_n<N, RK_C2R>(a, ny, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <int N>
+int ck_blocks_n() { return (N / 2) / CKShape<N>::RPW; }
+
+#define GSQG_FOR_EACH_N(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) X(8192)
+
+#if defined(GSQG_INSTANTIATE_N)
+template hipError_t ck_by_job<GSQG_INSTANTIATE_N>(int, const CKArgs&, int, hipStream_t);
+template hipError_t rk_by_job<GSQG_INSTANTIATE_N>(int, const RKArgs&, int, hipStream_t);
+#endif
+
+#if defined(GSQG_INSTANTIATE_COMMON)
+#define GSQG_EXTERN(N_)                                                              \
+  extern template hipError_t ck_by_job<N_>(int, const CKArgs&, int, hipStream_t); \
+  extern template hipError_t rk_by_job<N_>(int, const RKArgs&, int, hipStream_t);
+GSQG_FOR_EACH_N(GSQG_EXTERN)
+#undef GSQG_EXTERN
+
+#define GSQG_CASE(N_, CALL) \
+  case N_: return CALL;
+
+hipError_t launch_ck(int N, int job, const CKArgs& a, int ny, hipStream_t s) {
+  switch (N) {
+#define X(N_) GSQG_CASE(N_, ck_by_job<N_>(job, a, ny, s))
+    GSQG_FOR_EACH_N(X)
+#undef X
+  }
+  return hipErrorInvalidValue;
+}
+hipError_t launch_rk(int N, int job, const RKArgs& a, int ny, hipStream_t s) {
+  switch (N) {
+#define X(N_) GSQG_CASE(N_, rk_by_job<N_>(job, a, ny, s))
+    GSQG_FOR_EACH_N(X)
+#undef X
+  }
+  return hipErrorInvalidValue;
+}
+int ck_num_blocks(int N) {
+  switch (N) {
+#define X(N_) GSQG_CASE(N_, ck_blocks_n<N_>())
+    GSQG_FOR_EACH_N(X)
+#undef X
+  }
+  return 0;
+}
+
+hipError_t launch_full_to_half(const cd* X, cd* H, int N, double* norms, double* norm0,
+                               hipStream_t s) {
+  constexpr int T = 16;
+  if (N >= T) {
+    dim3 grid(N / T, ((N / 2) + T - 1) / T);
+    hipLaunchKernelGGL((full_to_half_kernel<T>), grid, dim3(256), 0, s, X, H, N, norms);
+  }
+  hipLaunchKernelGGL(full_to_half_row0_kernel, dim3(1), dim3(256), 0, s, X, H, N, norm0);
+  return hipGetLastError();
+}
+int full_to_half_num_blocks(int N) { return (N / 16) * (((N / 2) + 15) / 16); }
+
+hipError_t launch_half_to_full(const cd* H, cd* X, int N, hipStream_t s) {
+  constexpr int T = 16;
+  dim3 grid(N / T, N / T);
+  hipLaunchKernelGGL((half_to_full_kernel<T>), grid, dim3(256), 0, s, H, X, N);
+  return hipGetLastError();
+}
+
+hipError_t launch_a_gather(const cd* X, const int2* modes, cd* aval, int n, int N, double* norm,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(a_gather_kernel, dim3(1), dim3(256), 0, s, X, modes, aval, n, N, norm);
+  return hipGetLastError();
+}
+hipError_t launch_a_scatter(cd* X, const int2* modes, const cd* aval, int n, int N,
+                            hipStream_t s) {
+  int blocks = (n + 255) / 256;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(a_scatter_kernel, dim3(blocks), dim3(256), 0, s, X, modes, aval, n, N);
+  return hipGetLastError();
+}
+hipError_t launch_a_rhs(const Phys& ph, const double* k1d, const uint8_t* flags, const Ctrl* ctrl,
+                        const int2* modes, const cd* aval, cd* out, int n, int N, hipStream_t s) {
+  int blocks = (n + 255) / 256;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(a_rhs_kernel, dim3(blocks), dim3(256), 0, s, ph, k1d, flags, ctrl, modes,
+                     aval, out, n, N);
+  return hipGetLastError();
+}
+hipError_t launch_set_ctrl(Ctrl* c, int64_t step, uint64_t key, uint64_t seed, int key_mode,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(set_ctrl_kernel, dim3(1), dim3(1), 0, s, c, step, key, seed, key_mode);
+  return hipGetLastError();
+}
+
+
+#endif  // GSQG_INSTANTIATE_COMMON
+
+}  // namespace gsqg

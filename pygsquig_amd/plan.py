@@ -1,0 +1,215 @@
+"""Device plans: one ``gsqg_plan`` per (grid, physics) configuration.
+
+A plan owns the device work space of the fused step (half-packed ky-major
+fields, twiddle tables, the per-mode flag table, the ring side state) and one
+device-resident solver state.  This module builds the host tables the plan
+needs with the reference's operation order and wraps the C ABI.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import weakref
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from .grid import Grid, dealias_cutoff, wavenumbers_1d
+
+FLAG_DEALIAS = 1
+FLAG_LSDAMP = 2
+FLAG_RING = 4
+
+
+def half_plane_masks(N: int, L: float, damping_kf: Optional[float], ring: Optional[tuple]):
+    """Boolean masks over the half plane (N, N/2+1), [kx index][ky index],
+    computed as the reference computes them on the full plane:
+    dealias ``sqrt(k2) < (2/3) pi N/L`` (core/grid.py:102-105), large-scale
+    damping ``sqrt(k2) < kf/2`` (forcing/damping.py:34), ring
+    ``|sqrt(k2) - kf| <= dk/2`` (forcing/ring_forcing.py:74-75)."""
+    k = wavenumbers_1d(N, L)
+    kx = k[:, None]
+    ky = k[: N // 2 + 1].copy()
+    ky[N // 2] = k[N // 2]
+    ky = ky[None, :]
+    k2 = kx**2 + ky**2
+    kmag = np.sqrt(k2)
+    dealias = kmag < dealias_cutoff(N, L)
+    ls = (kmag < damping_kf / 2) if damping_kf is not None else np.zeros_like(dealias)
+    if ring is not None:
+        kf, dk = ring
+        rg = np.abs(kmag - kf) <= dk / 2
+    else:
+        rg = np.zeros_like(dealias)
+    return dealias, ls, rg
+
+
+def pack_flags(dealias, ls, rg) -> np.ndarray:
+    """Pack (N, N/2+1) masks into the plan's ky-major (N/2, N) flag bytes."""
+    N = dealias.shape[0]
+    full = (dealias.astype(np.uint8) | (ls.astype(np.uint8) << 1) | (rg.astype(np.uint8) << 2))
+    out = np.ascontiguousarray(full[:, : N // 2].T)
+    out[0, :] |= (full[:, N // 2] << 4).astype(np.uint8)
+    return out
+
+
+def _ptr(t: torch.Tensor):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Plan:
+    """Thin owner of a ``gsqg_plan*``."""
+
+    def __init__(self, grid: Grid, alpha: float = 1.0, nu_p: float = 0.0, p: int = 8, *,
+                 scheme: int = _lib.SCHEME_EXPLICIT, combine: int = _lib.COMBINE_STEP,
+                 hv_dc_zero: bool = True, damping=None, forcing=None,
+                 kappas: Sequence[float] = ()):
+        self.lib = _lib.load()
+        self.grid = grid
+        N, L = grid.N, grid.L
+        self.N = N
+        self.alpha = alpha
+        self.nu_p, self.p = nu_p, p
+        self.scheme, self.combine = scheme, combine
+        self.damping, self.forcing = damping, forcing
+        self.kappas = tuple(float(k) for k in kappas)
+        dk_kf = None
+        if damping is not None and damping.mu != 0:
+            dk_kf = damping.kf
+        ring = (forcing.kf, forcing.dk) if forcing is not None else None
+        dealias, ls, rg = half_plane_masks(N, L, dk_kf, ring)
+        self.flags = pack_flags(dealias, ls, rg)
+        self.ring_modes = np.ascontiguousarray(np.argwhere(rg).astype(np.int32)) if ring else \
+            np.zeros((0, 2), np.int32)
+        self.k1d = np.ascontiguousarray(wavenumbers_1d(N, L))
+        self._kappa_arr = (ctypes.c_double * max(1, len(self.kappas)))(*(self.kappas or (0.0,)))
+        d = _lib.GsqgPlanDesc()
+        d.n = N
+        d.L = L
+        d.alpha = alpha
+        d.nu_p = nu_p
+        d.p = p
+        d.scheme = scheme
+        d.combine = combine
+        d.hv_dc_zero = 1 if hv_dc_zero else 0
+        d.damping = 1 if damping is not None else 0
+        if damping is not None:
+            d.damp_mu = damping.mu
+            d.damp_nu_p = damping.nu_p
+            d.damp_p = damping.p
+        d.forcing = 1 if forcing is not None else 0
+        if forcing is not None:
+            if forcing.tau_f != 0:
+                raise NotImplementedError("the fused forcing path supports tau_f = 0 only")
+            d.epsilon = forcing.epsilon
+            d.amplitude = float(forcing.amplitude)
+        d.n_scalars = len(self.kappas)
+        d.kappa = self._kappa_arr
+        d.k1d = self.k1d.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        d.mode_flags = self.flags.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        d.ring_modes = self.ring_modes.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        d.n_ring = int(self.ring_modes.shape[0])
+        handle = ctypes.c_void_p()
+        with torch.cuda.device(grid.device):
+            _lib.check(self.lib.gsqg_plan_create(ctypes.byref(d), ctypes.byref(handle)),
+                       "gsqg_plan_create")
+        self.handle = handle
+        self._finalizer = weakref.finalize(self, Plan._destroy, self.lib, handle)
+        # identity of the full tensor mirroring the device state (see step())
+        self.state_ref = None
+        self.state_version = None
+        self.scalar_refs = {}
+
+    @staticmethod
+    def _destroy(lib, handle):
+        try:
+            lib.gsqg_plan_destroy(handle)
+        except Exception:  # pragma: no cover
+            pass
+
+    # --------------------------------------------------------------- state
+    def load_theta(self, th: torch.Tensor, check: bool = True) -> float:
+        off = ctypes.c_double(0.0)
+        _lib.check(self.lib.gsqg_load_theta(self.handle, _ptr(th), ctypes.byref(off) if check else None,
+                                            _stream()), "gsqg_load_theta")
+        self.state_ref = None
+        return off.value
+
+    def store_theta(self, out: torch.Tensor) -> None:
+        _lib.check(self.lib.gsqg_store_theta(self.handle, _ptr(out), _stream()), "gsqg_store_theta")
+
+    def load_scalar(self, idx: int, s: torch.Tensor, check: bool = True) -> float:
+        off = ctypes.c_double(0.0)
+        _lib.check(self.lib.gsqg_load_scalar(self.handle, idx, _ptr(s),
+                                             ctypes.byref(off) if check else None, _stream()),
+                   "gsqg_load_scalar")
+        self.scalar_refs.pop(idx, None)
+        return off.value
+
+    def store_scalar(self, idx: int, out: torch.Tensor) -> None:
+        _lib.check(self.lib.gsqg_store_scalar(self.handle, idx, _ptr(out), _stream()),
+                   "gsqg_store_scalar")
+
+    def advance(self, dt: float, n_steps: int, key_mode: int, key: int, step0: int = 0,
+                use_graph: bool = True) -> None:
+        _lib.check(self.lib.gsqg_advance(self.handle, float(dt), int(n_steps), int(key_mode),
+                                         ctypes.c_uint64(key & 0xFFFFFFFFFFFFFFFF),
+                                         int(step0), 1 if use_graph else 0, _stream()),
+                   "gsqg_advance")
+
+    def advance_scalars(self, u: torch.Tensor, v: torch.Tensor, dt: float) -> None:
+        _lib.check(self.lib.gsqg_advance_scalars(self.handle, _ptr(u), _ptr(v), float(dt),
+                                                 _stream()), "gsqg_advance_scalars")
+
+    # ----------------------------------------------------------- operators
+    def rhs(self, th: torch.Tensor, out: torch.Tensor, key: int = 0) -> None:
+        _lib.check(self.lib.gsqg_rhs(self.handle, _ptr(th), _ptr(out),
+                                     ctypes.c_uint64(key & 0xFFFFFFFFFFFFFFFF), _stream()),
+                   "gsqg_rhs")
+
+    def velocity(self, th: torch.Tensor, u: torch.Tensor, v: torch.Tensor) -> None:
+        _lib.check(self.lib.gsqg_velocity(self.handle, _ptr(th), _ptr(u), _ptr(v), _stream()),
+                   "gsqg_velocity")
+
+    def fft2(self, f: torch.Tensor, X: torch.Tensor) -> None:
+        _lib.check(self.lib.gsqg_fft2(self.handle, _ptr(f), _ptr(X), _stream()), "gsqg_fft2")
+
+    def ifft2(self, X: torch.Tensor, f: torch.Tensor) -> None:
+        _lib.check(self.lib.gsqg_ifft2(self.handle, _ptr(X), _ptr(f), _stream()), "gsqg_ifft2")
+
+    def nonfinite(self, clear: bool = True) -> bool:
+        flag = ctypes.c_int(0)
+        _lib.check(self.lib.gsqg_nonfinite(self.handle, ctypes.byref(flag), 1 if clear else 0,
+                                           _stream()), "gsqg_nonfinite")
+        return bool(flag.value)
+
+    def forcing_stats(self):
+        r = (ctypes.c_double * 4)()
+        s = (ctypes.c_double * 4)()
+        _lib.check(self.lib.gsqg_forcing_stats(self.handle, r, s, _stream()), "gsqg_forcing_stats")
+        return list(r), list(s)
+
+    def device_bytes(self) -> int:
+        b = ctypes.c_size_t(0)
+        _lib.check(self.lib.gsqg_plan_device_bytes(self.handle, ctypes.byref(b)), "device_bytes")
+        return b.value
+
+
+_TRANSFORM_PLANS: dict = {}
+_PLAN_CACHE_VEL: dict = {}
+
+
+def transform_plan(grid: Grid) -> Plan:
+    """Shared plan for fft2/ifft2 (only N matters for pure transforms)."""
+    key = (grid.N, grid.L, str(grid.device))
+    p = _TRANSFORM_PLANS.get(key)
+    if p is None:
+        p = Plan(grid, 1.0, 0.0, 8)
+        _TRANSFORM_PLANS[key] = p
+    return p
