@@ -74,6 +74,7 @@ struct gsqg_plan {
   int n_ring = 0;
   double* norms = nullptr;
   int n_norms = 0;
+  double2* row0 = nullptr;  // packed row-0 staging, N per y-slice
   size_t bytes = 0;
   // graph cache
   hipStream_t cap = nullptr;
@@ -97,7 +98,7 @@ static void free_plan(gsqg_plan* p) {
   void* ptrs[] = {p->th,   p->th2,   p->acc,  p->ys,   p->J,     p->F[0], p->F[1], p->F[2],
                   p->F[3], p->s_th,  p->s_acc, p->s_ys, p->s_J,  p->s_F0, p->s_F1, p->uphys,
                   p->vphys, p->k1d,  p->flags, p->tw,   p->ctrl, p->part, p->ring, p->a_val,
-                  p->a_tmp, p->norms};
+                  p->a_tmp, p->norms, p->row0};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
@@ -195,6 +196,7 @@ extern "C" int gsqg_plan_create(const gsqg_plan_desc* d, gsqg_plan** out) {
   ALLOC(p->a_tmp, (size_t)(p->n_ring > 0 ? p->n_ring : 1));
   p->n_norms = full_to_half_num_blocks(N) + 2;
   ALLOC(p->norms, (size_t)p->n_norms);
+  ALLOC(p->row0, (size_t)N * (size_t)(p->n_scalars > 0 ? p->n_scalars : 1));
 #undef ALLOC
 
   // Twiddle tables tw[T + k] = exp(-2 pi i k / T), long double on the host.
@@ -264,6 +266,7 @@ static CKArgs ck_theta(gsqg_plan* p, double2* th, int e, int prologue, int rate_
   a.a_val = p->a_val;
   a.n_a = p->n_ring;
   a.field_stride = 0;
+  a.row0 = p->row0;
   return a;
 }
 
@@ -320,6 +323,7 @@ static int launch_scalar_ck(gsqg_plan* p, int e, int prologue, double dt, hipStr
     a.F[1] = p->s_F1 + off;
     a.F[2] = a.F[3] = nullptr;
     a.ph.kappa = p->kappa[uni ? 0 : g];
+    a.row0 = p->row0 + (uni ? 0 : (size_t)g * p->N);
     HIPCHK(launch_ck(p->N, CK_SCALAR, a, uni ? p->n_scalars : 1, st));
   }
   return GSQG_OK;

@@ -141,9 +141,20 @@ __device__ __forceinline__ void fft_lanes(cd (&v)[8], cd* buf, int t, const cd* 
       for (int m = 0; m < 8; ++m) v[m] = buf[lpad(t + m * Q)];
     }
     if (Ns > 1) {
-      int jm = t % Ns;
-#pragma unroll
-      for (int r = 1; r < 8; ++r) v[r] = cmul(v[r], twiddle<DIR>(tw, Ns * 8, r * jm));
+      // w^r for r = 1..7 from three table reads (w, w^2, w^4) and four
+      // products; keeps the live twiddle registers low.
+      const int jm = t % Ns;
+      const cd w1 = twiddle<DIR>(tw, Ns * 8, jm);
+      const cd w2 = twiddle<DIR>(tw, Ns * 8, 2 * jm);
+      const cd w4 = twiddle<DIR>(tw, Ns * 8, 4 * jm);
+      const cd w3 = cmul(w1, w2);
+      v[1] = cmul(v[1], w1);
+      v[2] = cmul(v[2], w2);
+      v[3] = cmul(v[3], w3);
+      v[4] = cmul(v[4], w4);
+      v[5] = cmul(v[5], cmul(w1, w4));
+      v[6] = cmul(v[6], cmul(w2, w4));
+      v[7] = cmul(v[7], cmul(w3, w4));
     }
     dft8<DIR>(v);
     bool last = (p == P8 - 1) && (REM == 0);

@@ -210,28 +210,25 @@ __device__ __forceinline__ ModeOut mode_epilogue(int e, const ModeIn& in, double
 // Prologue multipliers (Hermitianised, core/operators.py:27-28,90-91 with
 // the .real of ifft2 folded in: i kx vanishes on the kx-Nyquist row and i ky
 // on the ky-Nyquist column), including the 1/N^2 of the inverse transform.
+// Field q of the job, given K = |k|^-alpha.
 template <int JOB>
-__device__ __forceinline__ void prologue_fields(cd y, double kx, double ky, double k2, bool kxn,
-                                                bool kyn, const Phys& ph, cd (&f)[4]) {
+__device__ __forceinline__ cd prologue_field(int q, cd y, double kx, double ky, double K, bool kxn,
+                                             bool kyn, const Phys& ph) {
   const double s = ph.inv_n2;
   const double cx = kxn ? 0.0 : kx * s;
   const double cy = kyn ? 0.0 : ky * s;
   if (JOB == CK_THETA) {
-    const double K = inv_frac(k2, ph);
-    f[0] = cmul_i(cscale(y, cx));          // theta_x
-    f[1] = cmul_i(cscale(y, cy));          // theta_y
-    f[2] = cmul_mi(cscale(y, cy * K));     // u = -d_y psi
-    f[3] = cmul_i(cscale(y, cx * K));      // v =  d_x psi
+    if (q == 0) return cmul_i(cscale(y, cx));        // theta_x
+    if (q == 1) return cmul_i(cscale(y, cy));        // theta_y
+    if (q == 2) return cmul_mi(cscale(y, cy * K));   // u = -d_y psi
+    return cmul_i(cscale(y, cx * K));                // v =  d_x psi
   } else if (JOB == CK_SCALAR) {
-    f[0] = cmul_i(cscale(y, cx));
-    f[1] = cmul_i(cscale(y, cy));
-    f[2] = f[3] = cmk(0.0, 0.0);
-  } else {  // CK_VEL
-    const double K = inv_frac(k2, ph);
-    f[0] = cmul_mi(cscale(y, cy * K));
-    f[1] = cmul_i(cscale(y, cx * K));
-    f[2] = f[3] = cmk(0.0, 0.0);
+    if (q == 0) return cmul_i(cscale(y, cx));
+    return cmul_i(cscale(y, cy));
   }
+  // CK_VEL
+  if (q == 0) return cmul_mi(cscale(y, cy * K));
+  return cmul_i(cscale(y, cx * K));
 }
 
 template <int JOB>
@@ -245,7 +242,91 @@ __device__ __forceinline__ unsigned mode_flags(const uint8_t* flags, int N, int 
 }
 
 // ------------------------------------------------------------ column kernel
-template <int N, int JOB>
+// Packed row 0 (ky = 0 and ky = N/2 of the reference's spectrum) couples kx
+// and -kx.  Lane t of a row owns kx indices t + m*Q; the partner of
+// (t, m) is (Q - t, 7 - m) for t > 0 and (0, (8 - m) mod 8) for t = 0, so
+// lanes t <= Q/2 process the element pairs of themselves and of lane Q - t:
+// every packed value is read and rewritten by exactly one lane.
+template <int Q>
+__device__ __forceinline__ int row0_pairs(int t, int (&mi)[8], int (&mp)[8]) {
+  // element index pairs (mine, partner's) handled by lane t; returns count
+  int n = 0;
+  if (t == 0) {
+    mi[n] = 0; mp[n] = 0; ++n;        // i = 0 (self-conjugate)
+    mi[n] = 4; mp[n] = 4; ++n;        // i = N/2 (self-conjugate)
+    for (int m = 1; m < 4; ++m) { mi[n] = m; mp[n] = 8 - m; ++n; }
+  } else if (2 * t == Q) {
+    for (int m = 0; m < 4; ++m) { mi[n] = m; mp[n] = 7 - m; ++n; }
+  } else if (2 * t < Q) {
+    for (int m = 0; m < 8; ++m) { mi[n] = m; mp[n] = 7 - m; ++n; }
+  }
+  return n;
+}
+
+template <int N>
+__device__ __forceinline__ int partner_lane(int t) {
+  constexpr int Q = N / 8;
+  return t == 0 ? 0 : Q - t;
+}
+
+// Epilogue of stage E at the two modes (i, 0) and (i, N/2) packed in Z(i),
+// given Z(i) and Z(-i) of every input.  Returns the packed outputs.
+template <int JOB, int E, bool FORCED>
+__device__ __forceinline__ void row0_epilogue(int i, const ModeIn& zi, const ModeIn& zp,
+                                              const double* k1d, const uint8_t* flags,
+                                              const Phys& ph, double dt, uint64_t key, double sc,
+                                              ModeOut& out) {
+  constexpr int N_ = 0;  // unused
+  (void)N_;
+  const int N = ph.N;
+  ModeOut oo[2];
+  const double kx = k1d[i];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int jy = h ? N / 2 : 0;
+    const double ky = k1d[jy];
+    const double k2 = kx * kx + ky * ky;
+    const unsigned fl = mode_flags(flags, N, i, jy);
+    ModeIn u;
+    u.J = h ? unpackN(zi.J, zp.J) : unpack0(zi.J, zp.J);
+    u.th = h ? unpackN(zi.th, zp.th) : unpack0(zi.th, zp.th);
+    u.acc = h ? unpackN(zi.acc, zp.acc) : unpack0(zi.acc, zp.acc);
+    u.ys = h ? unpackN(zi.ys, zp.ys) : unpack0(zi.ys, zp.ys);
+    cd sF = cmk(0.0, 0.0);
+    if (FORCED && (fl & FLAG_RING)) {
+      cd PF, QF;
+      ring_parts(key, i, jy, N, ph.amp, PF, QF);
+      sF = cscale(PF, sc);
+    }
+    const bool dc = (i == 0) && (jy == 0);
+    oo[h] = mode_epilogue<JOB == CK_SCALAR>(E, u, k2, dc, fl, ph, dt, FORCED, sF);
+  }
+  out.th = pack(oo[0].th, oo[1].th);
+  out.acc = pack(oo[0].acc, oo[1].acc);
+  out.ys = pack(oo[0].ys, oo[1].ys);
+  out.y = pack(oo[0].y, oo[1].y);
+}
+
+// <theta f> contribution of the packed row-0 element i (modes (i,0), (i,N/2)).
+__device__ __forceinline__ double row0_rate(int i, cd yi, cd yp, const uint8_t* flags,
+                                            const Phys& ph, uint64_t key) {
+  const int N = ph.N;
+  double r = 0.0;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int jy = h ? N / 2 : 0;
+    const unsigned fl = mode_flags(flags, N, i, jy);
+    if (fl & FLAG_RING) {
+      cd PF, QF;
+      ring_parts(key, i, jy, N, ph.amp, PF, QF);
+      const cd yy = h ? unpackN(yi, yp) : unpack0(yi, yp);
+      r += yy.x * PF.x + yy.y * PF.y;
+    }
+  }
+  return r;
+}
+
+template <int N, int JOB, int E, bool FORCED>
 __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
   using S = CKShape<N>;
   constexpr int Q = S::Q;
@@ -289,131 +370,117 @@ __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
 
   const Ctrl* ctrl = a.ctrl;
   const int64_t step = ctrl->step;
-  const bool forced = (JOB == CK_THETA) && ph.forcing;
+  uint64_t key = 0;
+  double sc = 1.0;
+  if (FORCED) {
+    key = ctrl_key(ctrl, step);
+    if (E > 0) sc = ctrl->scale[E];
+  }
+  // Row 0's packed prologue state is staged in global memory (row0 buffer).
+  cd* r0 = a.row0 + (size_t)blockIdx.y * N;
+  const cd* y0src = (E > 0) ? r0 : (a.th + fo);
 
-  // y[m] (packed in row 0) is the prologue state.
   cd y[8];
-  const int e = a.e;
-
-  if (e > 0) {
-    // Issue every load up front; the forward FFT covers their latency.
+  if (E > 0) {
     cd thv[8], accv[8], ysv[8];
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = a.J[ro + t + m * Q];
 #pragma unroll
     for (int m = 0; m < 8; ++m) thv[m] = a.th[ro + t + m * Q];
-    const bool need_acc = (e >= 2);
-    const bool need_ys = (ph.scheme == 0) && (e >= 3);
-    if (need_acc) {
+    constexpr bool need_acc = (E >= 2);
+    const bool need_ys = (ph.scheme == 0) && (E >= 3);
 #pragma unroll
-      for (int m = 0; m < 8; ++m) accv[m] = a.acc[ro + t + m * Q];
-    } else {
+    for (int m = 0; m < 8; ++m) accv[m] = need_acc ? a.acc[ro + t + m * Q] : cmk(0.0, 0.0);
 #pragma unroll
-      for (int m = 0; m < 8; ++m) accv[m] = cmk(0.0, 0.0);
-    }
-    if (need_ys) {
-#pragma unroll
-      for (int m = 0; m < 8; ++m) ysv[m] = a.ys[ro + t + m * Q];
-    } else {
-#pragma unroll
-      for (int m = 0; m < 8; ++m) ysv[m] = cmk(0.0, 0.0);
-    }
+    for (int m = 0; m < 8; ++m) ysv[m] = need_ys ? a.ys[ro + t + m * Q] : cmk(0.0, 0.0);
 
     fft_lanes<N, -1>(v, buf, t, tw);
 
-    double sc = 1.0;
-    uint64_t key = 0;
-    if (forced) {
-      sc = ctrl->scale[e];
-      key = ctrl_key(ctrl, step);
+    if (!row0) {
+      const int jy = j;
+      const double ky = k1d[jy];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const int i = t + m * Q;
+        const double kx = k1d[i];
+        const double k2 = kx * kx + ky * ky;
+        const unsigned fl = mode_flags(a.flags, N, i, jy);
+        ModeIn in;
+        in.J = v[m];
+        in.th = thv[m];
+        in.acc = accv[m];
+        in.ys = ysv[m];
+        cd sF = cmk(0.0, 0.0);
+        if (FORCED && (fl & FLAG_RING)) {
+          cd PF, QF;
+          ring_parts(key, i, jy, N, ph.amp, PF, QF);
+          sF = cscale(PF, sc);
+        }
+        ModeOut o = mode_epilogue<JOB == CK_SCALAR>(E, in, k2, false, fl, ph, a.dt, FORCED, sF);
+        if (E <= 3) a.acc[ro + i] = o.acc;
+        if ((E == 2 || E == 3) && ph.scheme == 0) a.ys[ro + i] = o.ys;
+        if (E == 4) a.th[ro + i] = o.th;
+        y[m] = o.y;
+      }
     }
-
     if (blk0) {
-      // Partner values X(-i) of packed row 0: the Jacobian through LDS, the
-      // state from global.  Barriers are taken by the whole of block 0.
+      // Row 0: the Jacobian goes through LDS, the rest is read from global
+      // by the lane that owns both partners (see row0_pairs).
       __syncthreads();
       if (row0) {
 #pragma unroll
         for (int m = 0; m < 8; ++m) buf[lpad(t + m * Q)] = v[m];
       }
       __syncthreads();
-    }
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const int i = t + m * Q;
-      const double kx = k1d[i];
-      ModeIn in;
-      in.J = v[m];
-      in.th = thv[m];
-      in.acc = accv[m];
-      in.ys = ysv[m];
-      if (!row0) {
-        const int jy = j;
-        const double ky = k1d[jy];
-        const double k2 = kx * kx + ky * ky;
-        const unsigned fl = mode_flags(a.flags, N, i, jy);
-        cd sF = cmk(0.0, 0.0);
-        if (forced && (fl & FLAG_RING)) {
-          cd PF, QF;
-          ring_parts(key, i, jy, N, ph.amp, PF, QF);
-          sF = cscale(PF, sc);
-        }
-        ModeOut o = mode_epilogue<JOB == CK_SCALAR>(e, in, k2, false, fl, ph, a.dt, forced, sF);
-        if (e == 1) a.acc[ro + i] = o.acc;
-        if (e == 2 || e == 3) {
-          a.acc[ro + i] = o.acc;
-          if (ph.scheme == 0) a.ys[ro + i] = o.ys;
-        }
-        if (e == 4) a.th[ro + i] = o.th;
-        y[m] = o.y;
-      } else {
-        const int ip = (N - i) & (N - 1);
-        const cd Jp = buf[lpad(ip)];
-        const cd thp = a.th[ro + ip];
-        const cd accp = need_acc ? a.acc[ro + ip] : cmk(0.0, 0.0);
-        const cd ysp = need_ys ? a.ys[ro + ip] : cmk(0.0, 0.0);
-        ModeOut oo[2];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int jy = h ? N / 2 : 0;
-          const double ky = k1d[jy];
-          const double k2 = kx * kx + ky * ky;
-          const unsigned fl = mode_flags(a.flags, N, i, jy);
-          ModeIn u;
-          u.J = h ? unpackN(in.J, Jp) : unpack0(in.J, Jp);
-          u.th = h ? unpackN(in.th, thp) : unpack0(in.th, thp);
-          u.acc = h ? unpackN(in.acc, accp) : unpack0(in.acc, accp);
-          u.ys = h ? unpackN(in.ys, ysp) : unpack0(in.ys, ysp);
-          cd sF = cmk(0.0, 0.0);
-          if (forced && (fl & FLAG_RING)) {
-            cd PF, QF;
-            ring_parts(key, i, jy, N, ph.amp, PF, QF);
-            sF = cscale(PF, sc);
+      if (row0) {
+        int mi[8], mp[8];
+        const int np = row0_pairs<Q>(t, mi, mp);
+        const int tp = partner_lane<N>(t);
+#pragma unroll 1
+        for (int q = 0; q < np; ++q) {
+          const int i = t + mi[q] * Q;
+          const int ip = tp + mp[q] * Q;
+          ModeIn zi, zp;
+          zi.J = buf[lpad(i)];
+          zp.J = buf[lpad(ip)];
+          zi.th = a.th[ro + i];
+          zp.th = a.th[ro + ip];
+          zi.acc = need_acc ? a.acc[ro + i] : cmk(0.0, 0.0);
+          zp.acc = need_acc ? a.acc[ro + ip] : cmk(0.0, 0.0);
+          zi.ys = need_ys ? a.ys[ro + i] : cmk(0.0, 0.0);
+          zp.ys = need_ys ? a.ys[ro + ip] : cmk(0.0, 0.0);
+          ModeOut oi, op;
+          row0_epilogue<JOB, E, FORCED>(i, zi, zp, k1d, a.flags, ph, a.dt, key, sc, oi);
+          if (ip != i)
+            row0_epilogue<JOB, E, FORCED>(ip, zp, zi, k1d, a.flags, ph, a.dt, key, sc, op);
+          if (E <= 3) a.acc[ro + i] = oi.acc;
+          if ((E == 2 || E == 3) && ph.scheme == 0) a.ys[ro + i] = oi.ys;
+          if (E == 4) a.th[ro + i] = oi.th;
+          r0[i] = oi.y;
+          if (ip != i) {
+            if (E <= 3) a.acc[ro + ip] = op.acc;
+            if ((E == 2 || E == 3) && ph.scheme == 0) a.ys[ro + ip] = op.ys;
+            if (E == 4) a.th[ro + ip] = op.th;
+            r0[ip] = op.y;
           }
-          const bool dc = (i == 0) && (jy == 0);
-          oo[h] = mode_epilogue<JOB == CK_SCALAR>(e, u, k2, dc, fl, ph, a.dt, forced, sF);
         }
-        if (e == 1) a.acc[ro + i] = pack(oo[0].acc, oo[1].acc);
-        if (e == 2 || e == 3) {
-          a.acc[ro + i] = pack(oo[0].acc, oo[1].acc);
-          if (ph.scheme == 0) a.ys[ro + i] = pack(oo[0].ys, oo[1].ys);
-        }
-        if (e == 4) {
-          a.th[ro + i] = pack(oo[0].th, oo[1].th);
-        }
-        y[m] = pack(oo[0].y, oo[1].y);
+      }
+      __threadfence_block();
+      __syncthreads();
+      if (row0) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) y[m] = r0[t + m * Q];
       }
     }
-    if (e == 4 && JOB == CK_THETA) {
+    if (E == 4 && JOB == CK_THETA) {
       bool bad = false;
 #pragma unroll
       for (int m = 0; m < 8; ++m) bad |= !(isfinite(y[m].x) && isfinite(y[m].y));
       if (bad) a.ctrl->nonfinite = 1.0;
     }
 
-    // Anti-Hermitian ring side state (only the final stage, theta only).
-    if (JOB == CK_THETA && e == 4 && forced && a.n_a > 0) {
-      const uint64_t k4 = key;
+    // Anti-Hermitian ring side state (final stage, theta only).
+    if (JOB == CK_THETA && E == 4 && FORCED) {
       const double s1 = ctrl->scale[1], s2 = ctrl->scale[2], s3 = ctrl->scale[3], s4 = ctrl->scale[4];
       for (int ent = blockIdx.x + gridDim.x * tid; ent < a.n_a; ent += gridDim.x * NT) {
         const int2 md = a.a_modes[ent];
@@ -423,7 +490,7 @@ __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
         const unsigned fl = mode_flags(a.flags, N, ii, jy);
         const bool dc = (ii == 0) && (jy == 0);
         cd PF, QF;
-        ring_parts(k4, ii, jy, N, ph.amp, PF, QF);
+        ring_parts(key, ii, jy, N, ph.amp, PF, QF);
         const cd A = a.a_val[ent];
         const cd zero = cmk(0.0, 0.0);
         cd An;
@@ -451,11 +518,10 @@ __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
         } else {
           const double L = lin_coeff(k2, dc, fl, ph);
           const double e1 = exp(L * 0.5 * a.dt), e2 = exp(L * a.dt);
-          cd k1 = cscale(QF, s1), k2v = cscale(QF, s2), k3 = cscale(QF, s3), k4v = cscale(QF, s4);
-          cd acc = cscale(k1, e2);
-          acc = cadd(acc, cscale(cscale(k2v, e1), 2.0));
-          acc = cadd(acc, cscale(cscale(k3, e1), 2.0));
-          An = cadd(cscale(A, e2), cscale(cadd(acc, k4v), a.dt / 6.0));
+          cd acc = cscale(cscale(QF, s1), e2);
+          acc = cadd(acc, cscale(cscale(cscale(QF, s2), e1), 2.0));
+          acc = cadd(acc, cscale(cscale(cscale(QF, s3), e1), 2.0));
+          An = cadd(cscale(A, e2), cscale(cadd(acc, cscale(QF, s4)), a.dt / 6.0));
         }
         a.a_val[ent] = An;
       }
@@ -470,48 +536,26 @@ __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
 
   // <theta f> partials of the prologue state (ring_forcing.py:153-155 by
   // Parseval over the ring modes).
-  if (forced && a.rate_slot > 0) {
-    const uint64_t key = ctrl_key(ctrl, a.rate_next_key ? step + 1 : step);
+  if (FORCED && a.rate_slot > 0) {
+    const uint64_t rkey = ctrl_key(ctrl, a.rate_next_key ? step + 1 : step);
     double accr = 0.0;
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const int i = t + m * Q;
-      if (!row0) {
-        const unsigned fl = mode_flags(a.flags, N, i, j);
-        if (fl & FLAG_RING) {
-          cd PF, QF;
-          ring_parts(key, i, j, N, ph.amp, PF, QF);
-          accr += 2.0 * (y[m].x * PF.x + y[m].y * PF.y);
-        }
-      } else {
-        // needs y(-i): stage through LDS below
-      }
-    }
-    if (blk0) {
-      __syncthreads();
-      if (row0) {
-#pragma unroll
-        for (int m = 0; m < 8; ++m) buf[lpad(t + m * Q)] = y[m];
-      }
-      __syncthreads();
-    }
-    if (row0) {
+    if (!row0) {
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         const int i = t + m * Q;
-        const int ip = (N - i) & (N - 1);
-        const cd yp = buf[lpad(ip)];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int jy = h ? N / 2 : 0;
-          const unsigned fl = mode_flags(a.flags, N, i, jy);
-          if (fl & FLAG_RING) {
-            cd PF, QF;
-            ring_parts(key, i, jy, N, ph.amp, PF, QF);
-            cd yy = h ? unpackN(y[m], yp) : unpack0(y[m], yp);
-            accr += yy.x * PF.x + yy.y * PF.y;
-          }
+        const unsigned fl = mode_flags(a.flags, N, i, j);
+        if (fl & FLAG_RING) {
+          cd PF, QF;
+          ring_parts(rkey, i, j, N, ph.amp, PF, QF);
+          accr += 2.0 * (y[m].x * PF.x + y[m].y * PF.y);
         }
+      }
+    } else {
+#pragma unroll 1
+      for (int m = 0; m < 8; ++m) {
+        const int i = t + m * Q;
+        const int ip = (N - i) & (N - 1);
+        accr += row0_rate(i, y0src[i], y0src[ip], a.flags, ph, rkey);
       }
     }
     red[tid] = accr;
@@ -524,42 +568,50 @@ __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
   }
 
   // Multipliers and inverse x-FFTs.
-  cd yp_row0[8];
-  if (blk0) {
-    __syncthreads();
-    if (row0) {
-#pragma unroll
-      for (int m = 0; m < 8; ++m) buf[lpad(t + m * Q)] = y[m];
-    }
-    __syncthreads();
-    if (row0) {
-#pragma unroll
-      for (int m = 0; m < 8; ++m) yp_row0[m] = buf[lpad((N - (t + m * Q)) & (N - 1))];
-    }
-  }
   constexpr int NF = NFields<JOB>::value;
-#pragma unroll
-  for (int q = 0; q < NF; ++q) {
+  double Kv[8];
+  {
+    const double ky = k1d[j];
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
-      const int i = t + m * Q;
-      const double kx = k1d[i];
-      const bool kxn = (i == N / 2);
-      cd f[4];
-      if (!row0) {
-        const double ky = k1d[j];
-        prologue_fields<JOB>(y[m], kx, ky, kx * kx + ky * ky, kxn, false, ph, f);
-        v[m] = f[q];
-      } else {
-        cd f0[4], fN[4];
-        const double kyN = k1d[N / 2];
-        prologue_fields<JOB>(unpack0(y[m], yp_row0[m]), kx, 0.0, kx * kx, kxn, false, ph, f0);
-        prologue_fields<JOB>(unpackN(y[m], yp_row0[m]), kx, kyN, kx * kx + kyN * kyN, kxn, true,
-                             ph, fN);
-        v[m] = pack(f0[q], fN[q]);
+      const double kx = k1d[t + m * Q];
+      Kv[m] = (JOB == CK_SCALAR) ? 0.0 : inv_frac(kx * kx + ky * ky, ph);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NF; ++q) {
+    if (!row0) {
+      const double ky = k1d[j];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const int i = t + m * Q;
+        v[m] = prologue_field<JOB>(q, y[m], k1d[i], ky, Kv[m], i == N / 2, false, ph);
       }
     }
-    __syncthreads();
+    if (blk0) {
+      __syncthreads();
+      if (row0) {
+        const double kyN = k1d[N / 2];
+#pragma unroll 1
+        for (int m = 0; m < 8; ++m) {
+          const int i = t + m * Q;
+          const int ip = (N - i) & (N - 1);
+          const cd yi = y0src[i], yp = y0src[ip];
+          const double kx = k1d[i];
+          const bool kxn = (i == N / 2);
+          const double K0 = (JOB == CK_SCALAR) ? 0.0 : inv_frac(kx * kx, ph);
+          const double KN = (JOB == CK_SCALAR) ? 0.0 : inv_frac(kx * kx + kyN * kyN, ph);
+          const cd f0 = prologue_field<JOB>(q, unpack0(yi, yp), kx, 0.0, K0, kxn, false, ph);
+          const cd fN = prologue_field<JOB>(q, unpackN(yi, yp), kx, kyN, KN, kxn, true, ph);
+          buf[lpad(i)] = pack(f0, fN);
+        }
+      }
+      __syncthreads();
+      if (row0) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) v[m] = buf[lpad(t + m * Q)];
+      }
+    }
     fft_lanes<N, +1>(v, buf, t, tw);
     cd* dst = a.F[q] + fo + (size_t)j * N;
 #pragma unroll
@@ -933,12 +985,24 @@ __global__ void set_ctrl_kernel(Ctrl* c, int64_t step, uint64_t key, uint64_t se
 // ------------------------------------------------------------- launchers
 // The build compiles this file once per grid size (GSQG_INSTANTIATE_N) and
 // once for the size-independent parts (GSQG_INSTANTIATE_COMMON).
-template <int N, int JOB>
+template <int N, int JOB, int E, bool FORCED>
 static hipError_t launch_ck_n(const CKArgs& a, int nfields_y, hipStream_t s) {
   using S = CKShape<N>;
   dim3 grid((N / 2) / S::RPW, nfields_y);
-  hipLaunchKernelGGL((ck_kernel<N, JOB>), grid, dim3(S::NT), 0, s, a);
+  hipLaunchKernelGGL((ck_kernel<N, JOB, E, FORCED>), grid, dim3(S::NT), 0, s, a);
   return hipGetLastError();
+}
+
+template <int N, int JOB, bool FORCED>
+static hipError_t launch_ck_e(const CKArgs& a, int ny, hipStream_t s) {
+  switch (a.e) {
+    case 0: return launch_ck_n<N, JOB, 0, FORCED>(a, ny, s);
+    case 1: return launch_ck_n<N, JOB, 1, FORCED>(a, ny, s);
+    case 2: return launch_ck_n<N, JOB, 2, FORCED>(a, ny, s);
+    case 3: return launch_ck_n<N, JOB, 3, FORCED>(a, ny, s);
+    case 4: return launch_ck_n<N, JOB, 4, FORCED>(a, ny, s);
+  }
+  return hipErrorInvalidValue;
 }
 
 template <int N, int JOB>
@@ -952,11 +1016,13 @@ static hipError_t launch_rk_n(const RKArgs& a, int nfields_y, hipStream_t s) {
 template <int N>
 hipError_t ck_by_job(int job, const CKArgs& a, int ny, hipStream_t s) {
   switch (job) {
-    case CK_THETA: return launch_ck_n<N, CK_THETA>(a, ny, s);
-    case CK_SCALAR: return launch_ck_n<N, CK_SCALAR>(a, ny, s);
-    case CK_VEL: return launch_ck_n<N, CK_VEL>(a, ny, s);
-    case CK_FWD: return launch_ck_n<N, CK_FWD>(a, ny, s);
-    case CK_INV: return launch_ck_n<N, CK_INV>(a, ny, s);
+    case CK_THETA:
+      return a.ph.forcing ? launch_ck_e<N, CK_THETA, true>(a, ny, s)
+                          : launch_ck_e<N, CK_THETA, false>(a, ny, s);
+    case CK_SCALAR: return launch_ck_e<N, CK_SCALAR, false>(a, ny, s);
+    case CK_VEL: return launch_ck_n<N, CK_VEL, 0, false>(a, ny, s);
+    case CK_FWD: return launch_ck_n<N, CK_FWD, 0, false>(a, ny, s);
+    case CK_INV: return launch_ck_n<N, CK_INV, 0, false>(a, ny, s);
   }
   return hipErrorInvalidValue;
 }
