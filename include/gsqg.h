@@ -142,6 +142,12 @@ int gsqg_velocity(gsqg_plan* plan, const void* theta_hat_dev, double* u_dev, dou
 int gsqg_fft2(gsqg_plan* plan, const double* field_dev, void* spectrum_dev, void* stream);
 int gsqg_ifft2(gsqg_plan* plan, const void* spectrum_dev, double* field_dev, void* stream);
 
+/* RingForcing.__call__(theta_hat, key) (forcing/ring_forcing.py:119-170):
+ * writes the rescaled forcing spectrum into the ring modes of out_dev, a full
+ * (n, n) complex128 array that the caller has zero-filled. */
+int gsqg_ring_forcing(gsqg_plan* plan, const void* theta_hat_dev, uint64_t key, void* out_dev,
+                      void* stream);
+
 /* Non-finite flag raised by the last steps (synchronises stream). */
 int gsqg_nonfinite(gsqg_plan* plan, int* flag, int clear, void* stream);
 

@@ -413,7 +413,13 @@ class Solver:
 
     def step(self, state, dt, forcing=None, damping=None):
         th = state["theta_hat"]
+        damp_obj = damping if isinstance(damping, CombinedDamping) else None
+        if damp_obj is not None:
+            damping = lambda y: damp_obj(y, self.grid)  # noqa: E731
         if self.scheme == "if":
+            if damping is not None and damp_obj is None:
+                raise ValueError("the IF scheme needs a CombinedDamping object")
+            damping = damp_obj
             lin = self.linear_operator(damping)
             g = self.grid
 

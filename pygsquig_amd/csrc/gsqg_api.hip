@@ -27,6 +27,9 @@ hipError_t launch_a_rhs(const Phys& ph, const double* k1d, const uint8_t* flags,
                         hipStream_t s);
 hipError_t launch_set_ctrl(Ctrl* c, int64_t step, uint64_t key, uint64_t seed, int key_mode,
                            hipStream_t s);
+hipError_t launch_ring_field(const double2* X, const int2* modes, int n, int N, double amp,
+                             double eps, uint64_t key, double2* out, double* rate_out,
+                             hipStream_t s);
 }  // namespace gsqg
 
 using namespace gsqg;
@@ -600,6 +603,16 @@ extern "C" int gsqg_ifft2(gsqg_plan* p, const void* X, double* f, void* stream) 
   r.tw = p->tw;
   r.ctrl = p->ctrl;
   HIPCHK(launch_rk(N, RK_C2R, r, 1, st));
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_ring_forcing(gsqg_plan* p, const void* X, uint64_t key, void* out,
+                                 void* stream) {
+  if (!p || !X || !out) return fail(GSQG_ERR_ARG, "null argument");
+  if (!p->ph.forcing) return fail(GSQG_ERR_ARG, "plan has no ring forcing");
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(launch_ring_field((const double2*)X, p->ring, p->n_ring, p->N, p->ph.amp, p->ph.eps, key,
+                           (double2*)out, p->norms, st));
   return GSQG_OK;
 }
 

@@ -973,6 +973,43 @@ __global__ void a_rhs_kernel(Phys ph, const double* k1d, const uint8_t* flags, c
   }
 }
 
+// RingForcing.__call__ on a full spectrum: rate by Parseval over the ring
+// list, then F = scale * amp * exp(i phi) at every ring mode
+// (ring_forcing.py:119-170).  One block; `out` must be zero-filled.
+__global__ void ring_field_kernel(const cd* __restrict__ X, const int2* modes, int n, int N,
+                                  double amp, double eps, uint64_t key, cd* out, double* rate_out) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int e = threadIdx.x; e < n; e += blockDim.x) {
+    const int i = modes[e].x, jy = modes[e].y;
+    const int mi = (N - i) & (N - 1), mj = (N - jy) & (N - 1);
+    const cd H = cscale(cadd(X[(size_t)i * N + jy], cconj(X[(size_t)mi * N + mj])), 0.5);
+    cd PF, QF;
+    ring_parts(key, i, jy, N, amp, PF, QF);
+    const double w = (jy == 0 || jy == N / 2) ? 1.0 : 2.0;
+    s += w * (H.x * PF.x + H.y * PF.y);
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  const double inv = 1.0 / ((double)N * (double)N);
+  const double rate = (red[0] * inv) * inv;
+  const double sc = fabs(rate) > 1e-10 ? eps / rate : 1.0;
+  if (threadIdx.x == 0 && rate_out) *rate_out = rate;
+  for (int e = threadIdx.x; e < n; e += blockDim.x) {
+    const int i = modes[e].x, jy = modes[e].y;
+    const int mi = (N - i) & (N - 1), mj = (N - jy) & (N - 1);
+    cd PF, QF;
+    ring_parts(key, i, jy, N, amp, PF, QF);
+    // F(k) = PF + QF, F(-k) = conj(PF - QF)
+    out[(size_t)i * N + jy] = cscale(cadd(PF, QF), sc);
+    if (jy != 0 && jy != N / 2) out[(size_t)mi * N + mj] = cscale(cconj(csub(PF, QF)), sc);
+  }
+}
+
 __global__ void set_ctrl_kernel(Ctrl* c, int64_t step, uint64_t key, uint64_t seed, int key_mode) {
   c->step = step;
   c->key = key;
@@ -1121,6 +1158,12 @@ hipError_t launch_a_rhs(const Phys& ph, const double* k1d, const uint8_t* flags,
   if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(a_rhs_kernel, dim3(blocks), dim3(256), 0, s, ph, k1d, flags, ctrl, modes,
                      aval, out, n, N);
+  return hipGetLastError();
+}
+hipError_t launch_ring_field(const cd* X, const int2* modes, int n, int N, double amp, double eps,
+                             uint64_t key, cd* out, double* rate_out, hipStream_t s) {
+  hipLaunchKernelGGL(ring_field_kernel, dim3(1), dim3(256), 0, s, X, modes, n, N, amp, eps, key,
+                     out, rate_out);
   return hipGetLastError();
 }
 hipError_t launch_set_ctrl(Ctrl* c, int64_t step, uint64_t key, uint64_t seed, int key_mode,

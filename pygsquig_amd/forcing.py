@@ -49,12 +49,12 @@ class RingForcing:
             raise TypeError("RingForcing.__call__ needs grid=")
         if self.tau_f != 0:
             raise NotImplementedError("tau_f > 0 (stateful OU forcing) is not supported")
-        from .plan import transform_plan
+        from .plan import forcing_plan
 
         k = key_to_u64(key) if key is not None else self.seed
-        th = theta_hat.to(device=grid.device, dtype=torch.complex128).contiguous()
-        out = torch.empty_like(th)
-        transform_plan(grid).ring_forcing(self, th, k, out)
+        th = torch.as_tensor(theta_hat).to(device=grid.device, dtype=torch.complex128).contiguous()
+        out = torch.zeros_like(th)
+        forcing_plan(grid, self).ring_forcing(th, k, out)
         return out
 
 

@@ -183,6 +183,11 @@ class Plan:
     def ifft2(self, X: torch.Tensor, f: torch.Tensor) -> None:
         _lib.check(self.lib.gsqg_ifft2(self.handle, _ptr(X), _ptr(f), _stream()), "gsqg_ifft2")
 
+    def ring_forcing(self, th: torch.Tensor, key: int, out: torch.Tensor) -> None:
+        _lib.check(self.lib.gsqg_ring_forcing(self.handle, _ptr(th),
+                                              ctypes.c_uint64(key & 0xFFFFFFFFFFFFFFFF),
+                                              _ptr(out), _stream()), "gsqg_ring_forcing")
+
     def nonfinite(self, clear: bool = True) -> bool:
         flag = ctypes.c_int(0)
         _lib.check(self.lib.gsqg_nonfinite(self.handle, ctypes.byref(flag), 1 if clear else 0,
@@ -203,6 +208,19 @@ class Plan:
 
 _TRANSFORM_PLANS: dict = {}
 _PLAN_CACHE_VEL: dict = {}
+
+
+_FORCING_PLANS: dict = {}
+
+
+def forcing_plan(grid: Grid, forcing) -> Plan:
+    """Plan carrying a RingForcing's ring list (for RingForcing.__call__)."""
+    key = (grid.N, grid.L, str(grid.device), forcing.signature())
+    p = _FORCING_PLANS.get(key)
+    if p is None:
+        p = Plan(grid, 1.0, 0.0, 8, forcing=forcing)
+        _FORCING_PLANS[key] = p
+    return p
 
 
 def transform_plan(grid: Grid) -> Plan:
