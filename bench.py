@@ -1,0 +1,178 @@
+"""Benchmark: the forced-dissipative gSQG RK4 step with two passive scalars.
+
+Workload (the configuration the headline metric is quoted on): N=1024,
+alpha=1, FP64; random-phase k^-5/3 initial theta (seed 2, rms 1); ring
+forcing kf=40, dk=2, eps=0.1 with per-step keys from seed 42; CombinedDamping
+mu=0.1 (|k| < 20), nu_p = 1e-8*(512/N)^8, p=8; solver nu_p = 0; two Gaussian
+passive scalars (sigma = 2pi/16); dt = 5e-4.  One "step" = one RK4 step of
+theta plus both scalars.  The hyperviscous term is integrated with the
+integrating-factor scheme: the explicit scheme is unstable at this nu (the
+timed state would be non-finite within tens of steps).
+
+Prints one JSON line; see the keys below.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM = 8.0e12  # B/s, MI355X HBM3E
+
+
+def step_bytes(N: int, n_s: int) -> float:
+    """Algorithmic HBM bytes of one RK4 step (3-pass 2D-FFT model):
+    60h + n_s (44h + 8r) + [n_s > 0] (5h + 2r), h = 16 N (N/2+1), r = 8 N^2."""
+    h = 16.0 * N * (N // 2 + 1)
+    r = 8.0 * N * N
+    return 60 * h + n_s * (44 * h + 8 * r) + (5 * h + 2 * r if n_s > 0 else 0.0)
+
+
+def launch_bytes(N: int, n_s: int) -> dict:
+    """Algorithmic bytes per launch of each kernel class: the theta column
+    kernel is pass 3 of one stage plus pass 1 of the next (5h + 5h), the row
+    kernel pass 2 (5h); per scalar the column kernel moves 5h + 3h and the row
+    kernel 3h + 2r."""
+    h = 16.0 * N * (N // 2 + 1)
+    r = 8.0 * N * N
+    return {"theta_column": 10 * h, "theta_row": 5 * h, "scalar_column": 8 * h * n_s,
+            "scalar_row": (3 * h + 2 * r) * n_s}
+
+
+def read_traffic(kernel: str):
+    """HBM bytes per launch measured with rocprofv3 PMC counters, from
+    profiles/pmc_traffic.txt (lines 'kernel_class bytes_per_launch')."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) >= 2 and parts[0] == kernel:
+                    return float(parts[1])
+    except OSError:
+        pass
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--N", type=int, default=1024)
+    ap.add_argument("--scheme", default="if", choices=["if", "explicit"])
+    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU path is "
+              "not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import pygsquig_amd as pg
+    from pygsquig_amd import _lib, ics
+    from pygsquig_amd.scalars import gSQGSolverWithScalars
+
+    N, dt, n_s = args.N, 5e-4, 2
+    L = 2 * np.pi
+    dev = torch.device("cuda:0")
+    g = pg.make_grid(N, L, dev)
+    forcing = pg.RingForcing(kf=40, dk=2, epsilon=0.1, seed=42)
+    damping = pg.CombinedDamping(mu=0.1, kf=40, nu_p=1e-8 * (512 / N) ** 8, p=8)
+    solver = gSQGSolverWithScalars(g, 1.0, 0.0, 8, scheme=args.scheme,
+                                   passive_scalars={f"s{i}": {"kappa": 0.0} for i in range(n_s)})
+    theta0 = torch.from_numpy(ics.kolmogorov_53(N, seed=2)).to(dev)
+    init = {f"s{i}": torch.from_numpy(ics.gaussian_blob(N, c, L / 16)).to(dev)
+            for i, c in enumerate(ics.SCALAR_CENTRES[:n_s])}
+    state = solver.initialize(theta0, scalar_init=init)
+    state = solver.multistep(state, args.warmup, dt, forcing=forcing, damping=damping)
+    torch.cuda.synchronize()
+
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    state = solver.multistep(state, args.steps, dt, forcing=forcing, damping=damping)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_s = e0.elapsed_time(e1) / 1e3
+    finite = bool(torch.isfinite(state["theta_hat"]).all())
+    sec = max(wall, gpu_s)
+    steps_per_s = args.steps / sec
+    value = N * N * steps_per_s
+    B = step_bytes(N, n_s)
+
+    # Live per-kernel timing (HIP events around every launch, no graph).
+    plan = solver._plan(forcing, damping)
+    prof = plan.profile_advance(dt, args.profile_steps, _lib.KEY_DERIVED, forcing.seed,
+                                int(state["step"]))
+    lb = launch_bytes(N, n_s)
+    dom = max((k for k in lb), key=lambda k: prof[k][0])
+    dom_ms, dom_n = prof[dom]
+    avg_s = dom_ms / 1e3 / max(dom_n, 1)
+    achieved = lb[dom] / avg_s / 1e9
+    kernel_table = {k: {"avg_us": round(prof[k][0] / max(prof[k][1], 1) * 1e3, 2),
+                        "launches": prof[k][1],
+                        "GBps": round(lb[k] / (prof[k][0] / 1e3 / max(prof[k][1], 1)) / 1e9, 1)
+                        if prof[k][1] else None}
+                    for k in lb}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import cpu_baseline
+
+        r = cpu_baseline.time_forced_scalars(N=N, n_steps=args.cpu_steps,
+                                             workers=cpu_baseline.host_cores(), n_scalars=n_s,
+                                             scheme=args.scheme, budget_s=25.0)
+        cpu = {"value": N * N / r["sec_per_step"], "unit": "grid-point RK4 steps/s",
+               "cores": r["workers"], "kind": "port",
+               "sample": f"{r['steps']} timed steps (after 1 warm-up) of the same N={N} forced "
+                         f"configuration with {n_s} scalars, NumPy/SciPy reference-structured "
+                         f"c2c path, scipy.fft workers={r['workers']}, CPU {r['cpu']}",
+               "sec_per_step": r["sec_per_step"]}
+
+    out = {
+        "metric": "N^2*RK4-steps/s + HBM GB/s at N=1024 fp64",
+        "value": value,
+        "unit": "grid-point RK4 steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"forced-dissipative gSQG N={N}, 2 passive scalars, dt=5e-4",
+                   "N": N, "alpha": 1.0, "scheme": args.scheme, "forcing": "ring kf=40 dk=2 eps=0.1",
+                   "damping": "mu=0.1, nu_p=1e-8*(512/N)^8, p=8", "scalars": n_s,
+                   "steps_per_s": steps_per_s, "finite": finite},
+        "hbm_GBps_algorithmic": B * steps_per_s / 1e9,
+        "step_roofline": {"bound": "hbm", "achieved": B * steps_per_s / 1e9, "peak": PEAK_HBM / 1e9,
+                          "unit": "GB/s", "frac": B * steps_per_s / PEAK_HBM,
+                          "bytes_per_step": B},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM / 1e9,
+                     "unit": "GB/s", "frac": achieved * 1e9 / PEAK_HBM,
+                     "traffic": read_traffic(dom), "bytes_per_launch": lb[dom],
+                     "avg_launch_us": avg_s * 1e6},
+        "kernels": kernel_table,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -148,6 +148,13 @@ int gsqg_ifft2(gsqg_plan* plan, const void* spectrum_dev, double* field_dev, voi
 int gsqg_ring_forcing(gsqg_plan* plan, const void* theta_hat_dev, uint64_t key, void* out_dev,
                       void* stream);
 
+/* gsqg_advance without graphs, with a HIP event pair around every kernel:
+ * class_ms[5] / class_count[5] receive the summed duration and the number
+ * of launches of the theta column, theta row, scalar column, scalar row and
+ * other kernels (synchronises stream). */
+int gsqg_profile_advance(gsqg_plan* plan, double dt, int n_steps, int key_mode, uint64_t key,
+                         int64_t step0, double* class_ms, int* class_count, void* stream);
+
 /* Non-finite flag raised by the last steps (synchronises stream). */
 int gsqg_nonfinite(gsqg_plan* plan, int* flag, int clear, void* stream);
 

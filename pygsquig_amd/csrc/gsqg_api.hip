@@ -79,6 +79,11 @@ struct gsqg_plan {
   int n_norms = 0;
   double2* row0 = nullptr;  // packed row-0 staging, N per y-slice
   size_t bytes = 0;
+  // live kernel timing (gsqg_profile_advance): events around every launch
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<int> ev_class;
+  size_t ev_used = 0;
   // graph cache
   hipStream_t cap = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -104,6 +109,7 @@ static void free_plan(gsqg_plan* p) {
                   p->a_tmp, p->norms, p->row0};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
+  for (hipEvent_t e : p->ev_pool) (void)hipEventDestroy(e);
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
   if (p->cap) (void)hipStreamDestroy(p->cap);
   delete p;
@@ -244,6 +250,45 @@ extern "C" int gsqg_plan_device_bytes(const gsqg_plan* p, size_t* bytes) {
   return GSQG_OK;
 }
 
+// ----------------------------------------------------------- launch wrappers
+// Kernel classes for live timing.
+enum { KC_THETA_COL = 0, KC_THETA_ROW = 1, KC_SCALAR_COL = 2, KC_SCALAR_ROW = 3, KC_OTHER = 4,
+       KC_COUNT = 5 };
+
+static hipError_t prof_mark(gsqg_plan* p, int cls, hipStream_t st) {
+  if (p->ev_used >= p->ev_pool.size()) {
+    hipEvent_t e;
+    hipError_t r = hipEventCreate(&e);
+    if (r != hipSuccess) return r;
+    p->ev_pool.push_back(e);
+    p->ev_class.push_back(cls);
+  }
+  p->ev_class[p->ev_used] = cls;
+  return hipEventRecord(p->ev_pool[p->ev_used++], st);
+}
+
+static hipError_t ck(gsqg_plan* p, int job, const CKArgs& a, int ny, hipStream_t st) {
+  const int cls = job == CK_THETA ? KC_THETA_COL : job == CK_SCALAR ? KC_SCALAR_COL : KC_OTHER;
+  if (p->prof) {
+    hipError_t r = prof_mark(p, cls, st);
+    if (r != hipSuccess) return r;
+  }
+  hipError_t r = launch_ck(p->N, job, a, ny, st);
+  if (r == hipSuccess && p->prof) r = prof_mark(p, -1, st);
+  return r;
+}
+
+static hipError_t rk(gsqg_plan* p, int job, const RKArgs& a, int ny, hipStream_t st) {
+  const int cls = job == RK_THETA ? KC_THETA_ROW : job == RK_SCALAR ? KC_SCALAR_ROW : KC_OTHER;
+  if (p->prof) {
+    hipError_t r = prof_mark(p, cls, st);
+    if (r != hipSuccess) return r;
+  }
+  hipError_t r = launch_rk(p->N, job, a, ny, st);
+  if (r == hipSuccess && p->prof) r = prof_mark(p, -1, st);
+  return r;
+}
+
 // ----------------------------------------------------------- arg builders
 static CKArgs ck_theta(gsqg_plan* p, double2* th, int e, int prologue, int rate_slot,
                        int rate_next, double dt) {
@@ -327,7 +372,7 @@ static int launch_scalar_ck(gsqg_plan* p, int e, int prologue, double dt, hipStr
     a.F[2] = a.F[3] = nullptr;
     a.ph.kappa = p->kappa[uni ? 0 : g];
     a.row0 = p->row0 + (uni ? 0 : (size_t)g * p->N);
-    HIPCHK(launch_ck(p->N, CK_SCALAR, a, uni ? p->n_scalars : 1, st));
+    HIPCHK(ck(p, CK_SCALAR, a, uni ? p->n_scalars : 1, st));
   }
   return GSQG_OK;
 }
@@ -343,7 +388,7 @@ static int launch_scalar_rk(gsqg_plan* p, const double* u, const double* v, hipS
   a.ctrl = p->ctrl;
   a.part = p->part;
   a.field_stride = (long long)p->hsz;
-  HIPCHK(launch_rk(p->N, RK_SCALAR, a, p->n_scalars, st));
+  HIPCHK(rk(p, RK_SCALAR, a, p->n_scalars, st));
   return GSQG_OK;
 }
 
@@ -367,10 +412,10 @@ static int theta_rest(gsqg_plan* p, double dt, bool derived, bool last_prologue,
                       hipStream_t st) {
   const int N = p->N;
   for (int e = 1; e <= 3; ++e) {
-    HIPCHK(launch_ck(N, CK_THETA, ck_theta(p, p->th, e, 1, e + 1, 0, dt), 1, st));
-    HIPCHK(launch_rk(N, RK_THETA, rk_theta(p, e + 1, 0, false), 1, st));
+    HIPCHK(ck(p, CK_THETA, ck_theta(p, p->th, e, 1, e + 1, 0, dt), 1, st));
+    HIPCHK(rk(p, RK_THETA, rk_theta(p, e + 1, 0, false), 1, st));
   }
-  HIPCHK(launch_ck(N, CK_THETA,
+  HIPCHK(ck(p, CK_THETA,
                    ck_theta(p, p->th, 4, last_prologue ? 1 : 0, last_rate, derived ? 1 : 0, dt),
                    1, st));
   return GSQG_OK;
@@ -382,7 +427,7 @@ static int step_body(gsqg_plan* p, double dt, bool derived, bool scalars_pending
                      bool last_prologue, int last_rate, bool scalar_next_prologue,
                      hipStream_t st) {
   const int N = p->N;
-  HIPCHK(launch_rk(N, RK_THETA, rk_theta(p, 1, derived ? 1 : 0, scalars_pending), 1, st));
+  HIPCHK(rk(p, RK_THETA, rk_theta(p, 1, derived ? 1 : 0, scalars_pending), 1, st));
   int rc;
   if (scalars_pending) {
     if ((rc = scalar_step(p, dt, scalar_next_prologue, st))) return rc;
@@ -402,14 +447,14 @@ extern "C" int gsqg_advance(gsqg_plan* p, double dt, int n_steps, int key_mode, 
   int rc;
   HIPCHK(launch_set_ctrl(p->ctrl, derived ? step0 - 1 : step0, key, key, derived ? 1 : 0, st));
   // Prologue of the first stage.
-  HIPCHK(launch_ck(N, CK_THETA, ck_theta(p, p->th, 0, 1, 1, derived ? 1 : 0, dt), 1, st));
+  HIPCHK(ck(p, CK_THETA, ck_theta(p, p->th, 0, 1, 1, derived ? 1 : 0, dt), 1, st));
   if (sc) {
     if ((rc = launch_scalar_ck(p, 0, 1, dt, st))) return rc;
   }
   // step 0
   {
     const bool last = (n_steps == 1);
-    HIPCHK(launch_rk(N, RK_THETA, rk_theta(p, 1, derived ? 1 : 0, false), 1, st));
+    HIPCHK(rk(p, RK_THETA, rk_theta(p, 1, derived ? 1 : 0, false), 1, st));
     if ((rc = theta_rest(p, dt, derived, !last || sc, last ? 0 : 1, st))) return rc;
   }
   // middle steps 1 .. n-2
@@ -453,7 +498,7 @@ extern "C" int gsqg_advance(gsqg_plan* p, double dt, int n_steps, int key_mode, 
     a.vout = p->vphys;
     a.tw = p->tw;
     a.ctrl = p->ctrl;
-    HIPCHK(launch_rk(N, RK_VEL, a, 1, st));
+    HIPCHK(rk(p, RK_VEL, a, 1, st));
     if ((rc = scalar_step(p, dt, false, st))) return rc;
   }
   return GSQG_OK;
@@ -533,12 +578,12 @@ extern "C" int gsqg_rhs(gsqg_plan* p, const void* X, void* out, uint64_t key, vo
   // compute_rhs is the explicit right-hand side whatever the plan's scheme.
   CKArgs c0 = ck_theta(p, p->th2, 0, 1, 1, 0, 0.0);
   c0.ph.scheme = 0;
-  HIPCHK(launch_ck(N, CK_THETA, c0, 1, st));
-  HIPCHK(launch_rk(N, RK_THETA, rk_theta(p, 1, 0, false), 1, st));
+  HIPCHK(ck(p, CK_THETA, c0, 1, st));
+  HIPCHK(rk(p, RK_THETA, rk_theta(p, 1, 0, false), 1, st));
   // Stage-1 epilogue writes k1 into acc (th2 is left untouched for e == 1).
   CKArgs c1 = ck_theta(p, p->th2, 1, 0, 0, 0, 0.0);
   c1.ph.scheme = 0;
-  HIPCHK(launch_ck(N, CK_THETA, c1, 1, st));
+  HIPCHK(ck(p, CK_THETA, c1, 1, st));
   HIPCHK(launch_half_to_full(p->acc, (double2*)out, N, st));
   if (p->n_ring > 0) {
     HIPCHK(launch_a_rhs(p->ph, p->k1d, p->flags, p->ctrl, p->ring, p->a_tmp, p->a_tmp, p->n_ring,
@@ -557,7 +602,7 @@ extern "C" int gsqg_velocity(gsqg_plan* p, const void* X, double* u, double* v, 
   CKArgs a = ck_theta(p, p->th2, 0, 1, 0, 0, 0.0);
   a.F[0] = p->F[2];
   a.F[1] = p->F[3];
-  HIPCHK(launch_ck(N, CK_VEL, a, 1, st));
+  HIPCHK(ck(p, CK_VEL, a, 1, st));
   RKArgs r{};
   r.F[0] = p->F[2];
   r.F[1] = p->F[3];
@@ -565,7 +610,7 @@ extern "C" int gsqg_velocity(gsqg_plan* p, const void* X, double* u, double* v, 
   r.vout = v;
   r.tw = p->tw;
   r.ctrl = p->ctrl;
-  HIPCHK(launch_rk(N, RK_VEL, r, 1, st));
+  HIPCHK(rk(p, RK_VEL, r, 1, st));
   return GSQG_OK;
 }
 
@@ -578,11 +623,11 @@ extern "C" int gsqg_fft2(gsqg_plan* p, const double* f, void* X, void* stream) {
   r.Jout = p->J;
   r.tw = p->tw;
   r.ctrl = p->ctrl;
-  HIPCHK(launch_rk(N, RK_R2C, r, 1, st));
+  HIPCHK(rk(p, RK_R2C, r, 1, st));
   CKArgs a = ck_theta(p, p->th2, 0, 0, 0, 0, 0.0);
   a.J = p->J;
   a.th = p->th2;
-  HIPCHK(launch_ck(N, CK_FWD, a, 1, st));
+  HIPCHK(ck(p, CK_FWD, a, 1, st));
   HIPCHK(launch_half_to_full(p->th2, (double2*)X, N, st));
   return GSQG_OK;
 }
@@ -596,13 +641,13 @@ extern "C" int gsqg_ifft2(gsqg_plan* p, const void* X, double* f, void* stream) 
   CKArgs a = ck_theta(p, p->th2, 0, 0, 0, 0, 0.0);
   a.th = p->th2;
   a.F[0] = p->F[0];
-  HIPCHK(launch_ck(N, CK_INV, a, 1, st));
+  HIPCHK(ck(p, CK_INV, a, 1, st));
   RKArgs r{};
   r.F[0] = p->F[0];
   r.phys_out = f;
   r.tw = p->tw;
   r.ctrl = p->ctrl;
-  HIPCHK(launch_rk(N, RK_C2R, r, 1, st));
+  HIPCHK(rk(p, RK_C2R, r, 1, st));
   return GSQG_OK;
 }
 
@@ -613,6 +658,33 @@ extern "C" int gsqg_ring_forcing(gsqg_plan* p, const void* X, uint64_t key, void
   hipStream_t st = (hipStream_t)stream;
   HIPCHK(launch_ring_field((const double2*)X, p->ring, p->n_ring, p->N, p->ph.amp, p->ph.eps, key,
                            (double2*)out, p->norms, st));
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_profile_advance(gsqg_plan* p, double dt, int n_steps, int key_mode,
+                                    uint64_t key, int64_t step0, double* class_ms,
+                                    int* class_count, void* stream) {
+  if (!p || !class_ms || !class_count) return fail(GSQG_ERR_ARG, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  p->prof = true;
+  p->ev_used = 0;
+  int rc = gsqg_advance(p, dt, n_steps, key_mode, key, step0, 0, stream);
+  p->prof = false;
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(st));
+  for (int c = 0; c < KC_COUNT; ++c) {
+    class_ms[c] = 0.0;
+    class_count[c] = 0;
+  }
+  for (size_t i = 0; i + 1 < p->ev_used; i += 2) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, p->ev_pool[i], p->ev_pool[i + 1]));
+    const int c = p->ev_class[i];
+    if (c >= 0 && c < KC_COUNT) {
+      class_ms[c] += ms;
+      class_count[c] += 1;
+    }
+  }
   return GSQG_OK;
 }
 

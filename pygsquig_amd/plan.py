@@ -188,6 +188,19 @@ class Plan:
                                               ctypes.c_uint64(key & 0xFFFFFFFFFFFFFFFF),
                                               _ptr(out), _stream()), "gsqg_ring_forcing")
 
+    KERNEL_CLASSES = ("theta_column", "theta_row", "scalar_column", "scalar_row", "other")
+
+    def profile_advance(self, dt: float, n_steps: int, key_mode: int, key: int, step0: int = 0):
+        """Advance with a HIP event pair around every kernel; returns
+        {class: (total_ms, launches)}."""
+        ms = (ctypes.c_double * 5)()
+        cnt = (ctypes.c_int * 5)()
+        _lib.check(self.lib.gsqg_profile_advance(self.handle, float(dt), int(n_steps), int(key_mode),
+                                                 ctypes.c_uint64(key & 0xFFFFFFFFFFFFFFFF),
+                                                 int(step0), ms, cnt, _stream()),
+                   "gsqg_profile_advance")
+        return {c: (ms[i], cnt[i]) for i, c in enumerate(self.KERNEL_CLASSES)}
+
     def nonfinite(self, clear: bool = True) -> bool:
         flag = ctypes.c_int(0)
         _lib.check(self.lib.gsqg_nonfinite(self.handle, ctypes.byref(flag), 1 if clear else 0,
