@@ -70,7 +70,20 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--copy-only", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (calibrates PMC byte counters)")
     args = ap.parse_args()
+    if args.copy_only:
+        import torch
+
+        n = args.copy_only * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 5}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU path is "
               "not part of this build", file=sys.stderr)
