@@ -55,6 +55,10 @@ __device__ __forceinline__ uint64_t phase_bits(uint64_t key, uint64_t idx) {
 __device__ __forceinline__ uint64_t step_key(uint64_t seed, int64_t n) {
   return mix64(seed + (uint64_t)(n + 1) * 0xD1B54A32D192ED03ull);
 }
+// u = (bits >> 11) * 2^-53 in [0, 1); the phase is phi = 2 pi u.
+__device__ __forceinline__ double phase_u(uint64_t key, uint64_t idx) {
+  return (double)(phase_bits(key, idx) >> 11) * 0x1.0p-53;
+}
 // phi = u * 2pi with u = (bits >> 11) * 2^-53.
 __device__ __forceinline__ double phase_of(uint64_t key, uint64_t idx) {
   const double two_pi = 6.283185307179586;  // binary64 nearest to 2*pi

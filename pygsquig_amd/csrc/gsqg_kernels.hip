@@ -98,8 +98,9 @@ __device__ __forceinline__ void ring_parts(uint64_t key, int i, int jy, int N, d
                                            cd& QF) {
   const int mi = (N - i) & (N - 1), mj = (N - jy) & (N - 1);
   double s1, c1, s2, c2;
-  sincos(phase_of(key, (uint64_t)i * (uint64_t)N + (uint64_t)jy), &s1, &c1);
-  sincos(phase_of(key, (uint64_t)mi * (uint64_t)N + (uint64_t)mj), &s2, &c2);
+  // exp(i phi) with phi = 2 pi u: sincospi(2u) needs no argument reduction
+  sincospi(2.0 * phase_u(key, (uint64_t)i * (uint64_t)N + (uint64_t)jy), &s1, &c1);
+  sincospi(2.0 * phase_u(key, (uint64_t)mi * (uint64_t)N + (uint64_t)mj), &s2, &c2);
   cd f1 = cmk(amp * c1, amp * s1);
   cd f2 = cmk(amp * c2, amp * s2);
   const bool selfc = (i == 0 || i == N / 2) && (jy == 0 || jy == N / 2);
@@ -242,88 +243,88 @@ __device__ __forceinline__ unsigned mode_flags(const uint8_t* flags, int N, int 
 }
 
 // ------------------------------------------------------------ column kernel
-// Packed row 0 (ky = 0 and ky = N/2 of the reference's spectrum) couples kx
-// and -kx.  Lane t of a row owns kx indices t + m*Q; the partner of
-// (t, m) is (Q - t, 7 - m) for t > 0 and (0, (8 - m) mod 8) for t = 0, so
-// lanes t <= Q/2 process the element pairs of themselves and of lane Q - t:
-// every packed value is read and rewritten by exactly one lane.
-template <int Q>
-__device__ __forceinline__ int row0_pairs(int t, int (&mi)[8], int (&mp)[8]) {
-  // element index pairs (mine, partner's) handled by lane t; returns count
-  int n = 0;
-  if (t == 0) {
-    mi[n] = 0; mp[n] = 0; ++n;        // i = 0 (self-conjugate)
-    mi[n] = 4; mp[n] = 4; ++n;        // i = N/2 (self-conjugate)
-    for (int m = 1; m < 4; ++m) { mi[n] = m; mp[n] = 8 - m; ++n; }
-  } else if (2 * t == Q) {
-    for (int m = 0; m < 4; ++m) { mi[n] = m; mp[n] = 7 - m; ++n; }
-  } else if (2 * t < Q) {
-    for (int m = 0; m < 8; ++m) { mi[n] = m; mp[n] = 7 - m; ++n; }
+// Row 0 of every spectral state array (theta, accumulator, stage input) is
+// kept in the "R layout": R[r] = X(r, 0) for 0 < r < N/2,
+// R[N/2 + r] = X(r, N/2) for 0 < r < N/2, and the four self-conjugate modes
+// as real pairs R[0] = X(0,0) + i X(0,N/2), R[N/2] = X(N/2,0) + i X(N/2,N/2).
+// Every element is then one mode (two real modes at r = 0, N/2) and is
+// updated in place by the lane that owns it.  The x transforms need the
+// packed form Z(i) = X(i,0) + i X(i,N/2); the conversion goes through LDS.
+
+// Mode (kx index, ky index) of R-layout element r (r != 0, N/2).
+__device__ __forceinline__ int2 r_mode(int r, int N) {
+  int2 m;
+  if (r < N / 2) {
+    m.x = r;
+    m.y = 0;
+  } else {
+    m.x = r - N / 2;
+    m.y = N / 2;
   }
-  return n;
+  return m;
 }
 
-template <int N>
-__device__ __forceinline__ int partner_lane(int t) {
-  constexpr int Q = N / 8;
-  return t == 0 ? 0 : Q - t;
+// Z (packed) from an R-layout row in LDS.
+__device__ __forceinline__ cd z_from_r(const cd* buf, int i, int N) {
+  if (i == 0 || i == N / 2) return buf[lpad(i)];
+  if (i < N / 2) return cadd(buf[lpad(i)], cmul_i(buf[lpad(N / 2 + i)]));
+  return cadd(cconj(buf[lpad(N - i)]), cmul_i(cconj(buf[lpad(N / 2 + N - i)])));
 }
 
-// Epilogue of stage E at the two modes (i, 0) and (i, N/2) packed in Z(i),
-// given Z(i) and Z(-i) of every input.  Returns the packed outputs.
-template <int JOB, int E, bool FORCED>
-__device__ __forceinline__ void row0_epilogue(int i, const ModeIn& zi, const ModeIn& zp,
-                                              const double* k1d, const uint8_t* flags,
-                                              const Phys& ph, double dt, uint64_t key, double sc,
-                                              ModeOut& out) {
-  constexpr int N_ = 0;  // unused
-  (void)N_;
+// R-layout element r from a packed Z row in LDS.
+__device__ __forceinline__ cd r_from_z(const cd* buf, int r, int N) {
+  if (r == 0 || r == N / 2) return buf[lpad(r)];
+  if (r < N / 2) return cscale(cadd(buf[lpad(r)], cconj(buf[lpad(N - r)])), 0.5);
+  const int q = r - N / 2;
+  return cscale(cmul_mi(csub(buf[lpad(q)], cconj(buf[lpad(N - q)]))), 0.5);
+}
+
+// Epilogue at one row-0 element in R layout.
+template <int JOB, bool FORCED>
+__device__ __forceinline__ ModeOut row0_elem_epilogue(int E, int r, const ModeIn& in,
+                                                      const double* k1d, const uint8_t* flags,
+                                                      const Phys& ph, double dt, uint64_t key,
+                                                      double sc) {
   const int N = ph.N;
-  ModeOut oo[2];
-  const double kx = k1d[i];
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int jy = h ? N / 2 : 0;
-    const double ky = k1d[jy];
-    const double k2 = kx * kx + ky * ky;
-    const unsigned fl = mode_flags(flags, N, i, jy);
-    ModeIn u;
-    u.J = h ? unpackN(zi.J, zp.J) : unpack0(zi.J, zp.J);
-    u.th = h ? unpackN(zi.th, zp.th) : unpack0(zi.th, zp.th);
-    u.acc = h ? unpackN(zi.acc, zp.acc) : unpack0(zi.acc, zp.acc);
-    u.ys = h ? unpackN(zi.ys, zp.ys) : unpack0(zi.ys, zp.ys);
+  if (r != 0 && r != N / 2) {
+    const int2 md = r_mode(r, N);
+    const double kx = k1d[md.x], ky = k1d[md.y];
+    const unsigned fl = mode_flags(flags, N, md.x, md.y);
     cd sF = cmk(0.0, 0.0);
     if (FORCED && (fl & FLAG_RING)) {
       cd PF, QF;
-      ring_parts(key, i, jy, N, ph.amp, PF, QF);
+      ring_parts(key, md.x, md.y, N, ph.amp, PF, QF);
       sF = cscale(PF, sc);
     }
-    const bool dc = (i == 0) && (jy == 0);
-    oo[h] = mode_epilogue<JOB == CK_SCALAR>(E, u, k2, dc, fl, ph, dt, FORCED, sF);
+    return mode_epilogue<JOB == CK_SCALAR>(E, in, kx * kx + ky * ky, false, fl, ph, dt, FORCED, sF);
   }
-  out.th = pack(oo[0].th, oo[1].th);
-  out.acc = pack(oo[0].acc, oo[1].acc);
-  out.ys = pack(oo[0].ys, oo[1].ys);
-  out.y = pack(oo[0].y, oo[1].y);
-}
-
-// <theta f> contribution of the packed row-0 element i (modes (i,0), (i,N/2)).
-__device__ __forceinline__ double row0_rate(int i, cd yi, cd yp, const uint8_t* flags,
-                                            const Phys& ph, uint64_t key) {
-  const int N = ph.N;
-  double r = 0.0;
+  // two real self-conjugate modes: (r, 0) in the real parts, (r, N/2) in the imaginary parts
+  ModeOut oo[2];
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int jy = h ? N / 2 : 0;
-    const unsigned fl = mode_flags(flags, N, i, jy);
-    if (fl & FLAG_RING) {
+    const double kx = k1d[r], ky = k1d[jy];
+    const unsigned fl = mode_flags(flags, N, r, jy);
+    ModeIn u;
+    u.J = cmk(h ? in.J.y : in.J.x, 0.0);
+    u.th = cmk(h ? in.th.y : in.th.x, 0.0);
+    u.acc = cmk(h ? in.acc.y : in.acc.x, 0.0);
+    u.ys = cmk(h ? in.ys.y : in.ys.x, 0.0);
+    cd sF = cmk(0.0, 0.0);
+    if (FORCED && (fl & FLAG_RING)) {
       cd PF, QF;
-      ring_parts(key, i, jy, N, ph.amp, PF, QF);
-      const cd yy = h ? unpackN(yi, yp) : unpack0(yi, yp);
-      r += yy.x * PF.x + yy.y * PF.y;
+      ring_parts(key, r, jy, N, ph.amp, PF, QF);
+      sF = cscale(PF, sc);
     }
+    oo[h] = mode_epilogue<JOB == CK_SCALAR>(E, u, kx * kx + ky * ky, (r == 0 && jy == 0), fl, ph,
+                                            dt, FORCED, sF);
   }
-  return r;
+  ModeOut o;
+  o.th = cmk(oo[0].th.x, oo[1].th.x);
+  o.acc = cmk(oo[0].acc.x, oo[1].acc.x);
+  o.ys = cmk(oo[0].ys.x, oo[1].ys.x);
+  o.y = cmk(oo[0].y.x, oo[1].y.x);
+  return o;
 }
 
 template <int N, int JOB, int E, bool FORCED>
@@ -357,11 +358,35 @@ __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = src[ro + t + m * Q];
     if (JOB == CK_INV) {
+      if (blk0) {  // row 0: R layout -> packed Z
+        __syncthreads();
+        if (row0) {
+#pragma unroll
+          for (int m = 0; m < 8; ++m) buf[lpad(t + m * Q)] = v[m];
+        }
+        __syncthreads();
+        if (row0) {
+#pragma unroll
+          for (int m = 0; m < 8; ++m) v[m] = z_from_r(buf, t + m * Q, N);
+        }
+      }
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[m] = cscale(v[m], ph.inv_n2);
       fft_lanes<N, +1>(v, buf, t, tw);
     } else {
       fft_lanes<N, -1>(v, buf, t, tw);
+      if (blk0) {  // row 0: packed Z -> R layout
+        __syncthreads();
+        if (row0) {
+#pragma unroll
+          for (int m = 0; m < 8; ++m) buf[lpad(t + m * Q)] = v[m];
+        }
+        __syncthreads();
+        if (row0) {
+#pragma unroll
+          for (int m = 0; m < 8; ++m) v[m] = r_from_z(buf, t + m * Q, N);
+        }
+      }
     }
 #pragma unroll
     for (int m = 0; m < 8; ++m) dst[ro + t + m * Q] = v[m];
@@ -376,19 +401,16 @@ __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
     key = ctrl_key(ctrl, step);
     if (E > 0) sc = ctrl->scale[E];
   }
-  // Row 0's packed prologue state is staged in global memory (row0 buffer).
-  cd* r0 = a.row0 + (size_t)blockIdx.y * N;
-  const cd* y0src = (E > 0) ? r0 : (a.th + fo);
 
   cd y[8];
   if (E > 0) {
     cd thv[8], accv[8], ysv[8];
+    constexpr bool need_acc = (E >= 2);
+    const bool need_ys = (ph.scheme == 0) && (E >= 3);
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = a.J[ro + t + m * Q];
 #pragma unroll
     for (int m = 0; m < 8; ++m) thv[m] = a.th[ro + t + m * Q];
-    constexpr bool need_acc = (E >= 2);
-    const bool need_ys = (ph.scheme == 0) && (E >= 3);
 #pragma unroll
     for (int m = 0; m < 8; ++m) accv[m] = need_acc ? a.acc[ro + t + m * Q] : cmk(0.0, 0.0);
 #pragma unroll
@@ -396,36 +418,7 @@ __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
 
     fft_lanes<N, -1>(v, buf, t, tw);
 
-    if (!row0) {
-      const int jy = j;
-      const double ky = k1d[jy];
-#pragma unroll
-      for (int m = 0; m < 8; ++m) {
-        const int i = t + m * Q;
-        const double kx = k1d[i];
-        const double k2 = kx * kx + ky * ky;
-        const unsigned fl = mode_flags(a.flags, N, i, jy);
-        ModeIn in;
-        in.J = v[m];
-        in.th = thv[m];
-        in.acc = accv[m];
-        in.ys = ysv[m];
-        cd sF = cmk(0.0, 0.0);
-        if (FORCED && (fl & FLAG_RING)) {
-          cd PF, QF;
-          ring_parts(key, i, jy, N, ph.amp, PF, QF);
-          sF = cscale(PF, sc);
-        }
-        ModeOut o = mode_epilogue<JOB == CK_SCALAR>(E, in, k2, false, fl, ph, a.dt, FORCED, sF);
-        if (E <= 3) a.acc[ro + i] = o.acc;
-        if ((E == 2 || E == 3) && ph.scheme == 0) a.ys[ro + i] = o.ys;
-        if (E == 4) a.th[ro + i] = o.th;
-        y[m] = o.y;
-      }
-    }
-    if (blk0) {
-      // Row 0: the Jacobian goes through LDS, the rest is read from global
-      // by the lane that owns both partners (see row0_pairs).
+    if (blk0) {  // row 0: forward-transformed Jacobian, packed Z -> R layout
       __syncthreads();
       if (row0) {
 #pragma unroll
@@ -433,44 +426,39 @@ __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
       }
       __syncthreads();
       if (row0) {
-        int mi[8], mp[8];
-        const int np = row0_pairs<Q>(t, mi, mp);
-        const int tp = partner_lane<N>(t);
-#pragma unroll 1
-        for (int q = 0; q < np; ++q) {
-          const int i = t + mi[q] * Q;
-          const int ip = tp + mp[q] * Q;
-          ModeIn zi, zp;
-          zi.J = buf[lpad(i)];
-          zp.J = buf[lpad(ip)];
-          zi.th = a.th[ro + i];
-          zp.th = a.th[ro + ip];
-          zi.acc = need_acc ? a.acc[ro + i] : cmk(0.0, 0.0);
-          zp.acc = need_acc ? a.acc[ro + ip] : cmk(0.0, 0.0);
-          zi.ys = need_ys ? a.ys[ro + i] : cmk(0.0, 0.0);
-          zp.ys = need_ys ? a.ys[ro + ip] : cmk(0.0, 0.0);
-          ModeOut oi, op;
-          row0_epilogue<JOB, E, FORCED>(i, zi, zp, k1d, a.flags, ph, a.dt, key, sc, oi);
-          if (ip != i)
-            row0_epilogue<JOB, E, FORCED>(ip, zp, zi, k1d, a.flags, ph, a.dt, key, sc, op);
-          if (E <= 3) a.acc[ro + i] = oi.acc;
-          if ((E == 2 || E == 3) && ph.scheme == 0) a.ys[ro + i] = oi.ys;
-          if (E == 4) a.th[ro + i] = oi.th;
-          r0[i] = oi.y;
-          if (ip != i) {
-            if (E <= 3) a.acc[ro + ip] = op.acc;
-            if ((E == 2 || E == 3) && ph.scheme == 0) a.ys[ro + ip] = op.ys;
-            if (E == 4) a.th[ro + ip] = op.th;
-            r0[ip] = op.y;
-          }
-        }
-      }
-      __threadfence_block();
-      __syncthreads();
-      if (row0) {
 #pragma unroll
-        for (int m = 0; m < 8; ++m) y[m] = r0[t + m * Q];
+        for (int m = 0; m < 8; ++m) v[m] = r_from_z(buf, t + m * Q, N);
       }
+    }
+
+    const double kyj = k1d[j];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int i = t + m * Q;
+      ModeIn in;
+      in.J = v[m];
+      in.th = thv[m];
+      in.acc = accv[m];
+      in.ys = ysv[m];
+      ModeOut o;
+      if (!row0) {
+        const double kx = k1d[i];
+        const unsigned fl = mode_flags(a.flags, N, i, j);
+        cd sF = cmk(0.0, 0.0);
+        if (FORCED && (fl & FLAG_RING)) {
+          cd PF, QF;
+          ring_parts(key, i, j, N, ph.amp, PF, QF);
+          sF = cscale(PF, sc);
+        }
+        o = mode_epilogue<JOB == CK_SCALAR>(E, in, kx * kx + kyj * kyj, false, fl, ph, a.dt,
+                                            FORCED, sF);
+      } else {
+        o = row0_elem_epilogue<JOB, FORCED>(E, i, in, k1d, a.flags, ph, a.dt, key, sc);
+      }
+      if (E <= 3) a.acc[ro + i] = o.acc;
+      if ((E == 2 || E == 3) && ph.scheme == 0) a.ys[ro + i] = o.ys;
+      if (E == 4) a.th[ro + i] = o.th;
+      y[m] = o.y;
     }
     if (E == 4 && JOB == CK_THETA) {
       bool bad = false;
@@ -535,27 +523,39 @@ __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
   if (!a.prologue) return;
 
   // <theta f> partials of the prologue state (ring_forcing.py:153-155 by
-  // Parseval over the ring modes).
+  // Parseval over the ring modes; weight 2 for each +/-k pair).
   if (FORCED && a.rate_slot > 0) {
     const uint64_t rkey = ctrl_key(ctrl, a.rate_next_key ? step + 1 : step);
     double accr = 0.0;
-    if (!row0) {
 #pragma unroll
-      for (int m = 0; m < 8; ++m) {
-        const int i = t + m * Q;
+    for (int m = 0; m < 8; ++m) {
+      const int i = t + m * Q;
+      if (!row0) {
         const unsigned fl = mode_flags(a.flags, N, i, j);
         if (fl & FLAG_RING) {
           cd PF, QF;
           ring_parts(rkey, i, j, N, ph.amp, PF, QF);
           accr += 2.0 * (y[m].x * PF.x + y[m].y * PF.y);
         }
-      }
-    } else {
-#pragma unroll 1
-      for (int m = 0; m < 8; ++m) {
-        const int i = t + m * Q;
-        const int ip = (N - i) & (N - 1);
-        accr += row0_rate(i, y0src[i], y0src[ip], a.flags, ph, rkey);
+      } else if (i != 0 && i != N / 2) {
+        const int2 md = r_mode(i, N);
+        const unsigned fl = mode_flags(a.flags, N, md.x, md.y);
+        if (fl & FLAG_RING) {
+          cd PF, QF;
+          ring_parts(rkey, md.x, md.y, N, ph.amp, PF, QF);
+          accr += 2.0 * (y[m].x * PF.x + y[m].y * PF.y);
+        }
+      } else {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int jy = h ? N / 2 : 0;
+          const unsigned fl = mode_flags(a.flags, N, i, jy);
+          if (fl & FLAG_RING) {
+            cd PF, QF;
+            ring_parts(rkey, i, jy, N, ph.amp, PF, QF);
+            accr += (h ? y[m].y : y[m].x) * PF.x;
+          }
+        }
       }
     }
     red[tid] = accr;
@@ -567,49 +567,37 @@ __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
     if (tid == 0) a.part[(size_t)a.rate_slot * a.part_stride + blockIdx.x] = red[0];
   }
 
-  // Multipliers and inverse x-FFTs.
+  // Multipliers (per mode; row 0 in R layout) and inverse x-FFTs.
   constexpr int NF = NFields<JOB>::value;
-  double Kv[8];
-  {
-    const double ky = k1d[j];
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const double kx = k1d[t + m * Q];
-      Kv[m] = (JOB == CK_SCALAR) ? 0.0 : inv_frac(kx * kx + ky * ky, ph);
-    }
-  }
 #pragma unroll
   for (int q = 0; q < NF; ++q) {
-    if (!row0) {
-      const double ky = k1d[j];
+    const bool needK = (JOB != CK_SCALAR) && (JOB == CK_VEL || q >= 2);
 #pragma unroll
-      for (int m = 0; m < 8; ++m) {
-        const int i = t + m * Q;
-        v[m] = prologue_field<JOB>(q, y[m], k1d[i], ky, Kv[m], i == N / 2, false, ph);
+    for (int m = 0; m < 8; ++m) {
+      const int i = t + m * Q;
+      int2 md;
+      if (!row0) {
+        md.x = i;
+        md.y = j;
+      } else {
+        md = r_mode(i, N);  // the self-conjugate modes give zero fields
       }
+      const double kx = k1d[md.x], ky = k1d[md.y];
+      const double K = needK ? inv_frac(kx * kx + ky * ky, ph) : 0.0;
+      const bool selfc = row0 && (i == 0 || i == N / 2);
+      v[m] = selfc ? cmk(0.0, 0.0)
+                   : prologue_field<JOB>(q, y[m], kx, ky, K, md.x == N / 2, md.y == N / 2, ph);
     }
-    if (blk0) {
+    if (blk0) {  // row 0: R layout -> packed Z
       __syncthreads();
       if (row0) {
-        const double kyN = k1d[N / 2];
-#pragma unroll 1
-        for (int m = 0; m < 8; ++m) {
-          const int i = t + m * Q;
-          const int ip = (N - i) & (N - 1);
-          const cd yi = y0src[i], yp = y0src[ip];
-          const double kx = k1d[i];
-          const bool kxn = (i == N / 2);
-          const double K0 = (JOB == CK_SCALAR) ? 0.0 : inv_frac(kx * kx, ph);
-          const double KN = (JOB == CK_SCALAR) ? 0.0 : inv_frac(kx * kx + kyN * kyN, ph);
-          const cd f0 = prologue_field<JOB>(q, unpack0(yi, yp), kx, 0.0, K0, kxn, false, ph);
-          const cd fN = prologue_field<JOB>(q, unpackN(yi, yp), kx, kyN, KN, kxn, true, ph);
-          buf[lpad(i)] = pack(f0, fN);
-        }
+#pragma unroll
+        for (int m = 0; m < 8; ++m) buf[lpad(t + m * Q)] = v[m];
       }
       __syncthreads();
       if (row0) {
 #pragma unroll
-        for (int m = 0; m < 8; ++m) v[m] = buf[lpad(t + m * Q)];
+        for (int m = 0; m < 8; ++m) v[m] = z_from_r(buf, t + m * Q, N);
       }
     }
     fft_lanes<N, +1>(v, buf, t, tw);
@@ -865,19 +853,30 @@ __global__ void full_to_half_kernel(const cd* __restrict__ X, cd* __restrict__ H
   if (threadIdx.x == 0 && norms) norms[blockIdx.y * gridDim.x + blockIdx.x] = red[0];
 }
 
-// Row 0 of the half-packed layout (ky = 0 and ky = N/2 columns).
+// Row 0 of the spectral layout (ky = 0 and ky = N/2 columns) in R layout:
+// R[r] = P(X)(r, 0), R[N/2 + r] = P(X)(r, N/2), self-conjugate modes as
+// real pairs at r = 0 and r = N/2.
 __global__ void full_to_half_row0_kernel(const cd* __restrict__ X, cd* __restrict__ H, int N,
                                          double* norm0) {
   __shared__ double red[256];
   double nrm = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < N; r += gridDim.x * blockDim.x) {
+    const int i = (r <= N / 2) ? r : r - N / 2;   // kx index
+    const int jy = (r < N / 2 || r == 0) ? 0 : (r == N / 2 ? 0 : N / 2);
     const int mi = (N - i) & (N - 1);
-    const cd a0 = X[(size_t)i * N], b0 = cconj(X[(size_t)mi * N]);
-    const cd aN = X[(size_t)i * N + N / 2], bN = cconj(X[(size_t)mi * N + N / 2]);
-    const cd p0 = cscale(cadd(a0, b0), 0.5), pN = cscale(cadd(aN, bN), 0.5);
-    H[i] = pack(p0, pN);
-    const cd q0 = cscale(csub(a0, b0), 0.5), qN = cscale(csub(aN, bN), 0.5);
-    nrm += q0.x * q0.x + q0.y * q0.y + qN.x * qN.x + qN.y * qN.y;
+    if (r == 0 || r == N / 2) {
+      const cd a0 = X[(size_t)i * N], b0 = cconj(X[(size_t)mi * N]);
+      const cd aN = X[(size_t)i * N + N / 2], bN = cconj(X[(size_t)mi * N + N / 2]);
+      const cd p0 = cscale(cadd(a0, b0), 0.5), pN = cscale(cadd(aN, bN), 0.5);
+      H[r] = cmk(p0.x, pN.x);
+      const cd q0 = cscale(csub(a0, b0), 0.5), qN = cscale(csub(aN, bN), 0.5);
+      nrm += q0.x * q0.x + q0.y * q0.y + qN.x * qN.x + qN.y * qN.y;
+    } else {
+      const cd a = X[(size_t)i * N + jy], b = cconj(X[(size_t)mi * N + jy]);
+      H[r] = cscale(cadd(a, b), 0.5);
+      const cd q = cscale(csub(a, b), 0.5);
+      nrm += 2.0 * (q.x * q.x + q.y * q.y);  // the mirror mode -i is not visited
+    }
   }
   red[threadIdx.x] = nrm;
   __syncthreads();
@@ -906,8 +905,14 @@ __global__ void half_to_full_kernel(const cd* __restrict__ H, cd* __restrict__ X
     } else if (jy > N / 2) {
       val = cconj(H[(size_t)(N - jy) * N + ((N - i) & (N - 1))]);
     } else {
-      const cd z = H[i], zp = H[(N - i) & (N - 1)];
-      val = (jy == 0) ? unpack0(z, zp) : unpackN(z, zp);
+      // row 0, R layout
+      if (i == 0 || i == N / 2) {
+        const cd R = H[i];
+        val = cmk(jy == 0 ? R.x : R.y, 0.0);
+      } else {
+        const int base = (jy == 0) ? 0 : N / 2;
+        val = (i < N / 2) ? H[base + i] : cconj(H[base + N - i]);
+      }
     }
     a[tx][r] = val;
   }
