@@ -11,11 +11,9 @@ Modules
 reference_sqg
     NumPy FP64 restatement of pygSQuiG's step with the reference's own data
     layout: full complex (N, N) spectra, c2c FFTs, ``ifft2(...).real``.  Every
-    function cites the reference file:line it restates.
-half_sqg
-    The same mathematics in the layout the GPU kernels use (half spectrum plus
-    the anti-Hermitian ring side state) and the integrating-factor scheme.
-    Proven equal to ``reference_sqg`` by the tests.
+    function cites the reference file:line it restates.  Also holds the
+    integrating-factor RK4 scheme (new; no reference counterpart), checked by
+    its exact linear decay and its convergence to the explicit scheme.
 rng
     The counter-based generator used for forcing phases, shared bit-for-bit
     with the HIP kernels (integer arithmetic, pinned by known-answer vectors).

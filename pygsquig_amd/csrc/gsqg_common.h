@@ -14,6 +14,7 @@
 namespace gsqg {
 
 using cd = double2;
+typedef double vd2 __attribute__((vector_size(16)));  // native 2-vector, same bytes as cd
 
 __device__ __forceinline__ cd cmk(double a, double b) {
   cd r;
@@ -58,12 +59,6 @@ __device__ __forceinline__ uint64_t step_key(uint64_t seed, int64_t n) {
 // u = (bits >> 11) * 2^-53 in [0, 1); the phase is phi = 2 pi u.
 __device__ __forceinline__ double phase_u(uint64_t key, uint64_t idx) {
   return (double)(phase_bits(key, idx) >> 11) * 0x1.0p-53;
-}
-// phi = u * 2pi with u = (bits >> 11) * 2^-53.
-__device__ __forceinline__ double phase_of(uint64_t key, uint64_t idx) {
-  const double two_pi = 6.283185307179586;  // binary64 nearest to 2*pi
-  double u = (double)(phase_bits(key, idx) >> 11) * 0x1.0p-53;
-  return u * two_pi;
 }
 
 // ---------------------------------------------------------------- FFT
@@ -202,6 +197,102 @@ __device__ __forceinline__ void fft_lanes(cd (&v)[8], cd* buf, int t, const cd* 
       v[q + 2] = a1;
       v[q + 4] = a2;
       v[q + 6] = a3;
+    }
+  }
+}
+
+// G independent length-L transforms advanced in lockstep: transform g keeps
+// its elements in v[g] (same ownership as fft_lanes) and exchanges through
+// the LDS region buf + g * gs.  The G transforms share one set of barriers
+// and one set of twiddle reads, so a wave has G times the independent work
+// between barriers.
+template <int L, int DIR, int G>
+__device__ __forceinline__ void fft_lanes_multi(cd (&v)[G][8], cd* buf, int gs, int t,
+                                                const cd* __restrict__ tw) {
+  static_assert(L >= 8 && (L & (L - 1)) == 0, "power of two >= 8");
+  constexpr int Q = L / 8;
+  constexpr int LOG = __builtin_ctz(L);
+  constexpr int P8 = LOG / 3;
+  constexpr int REM = LOG % 3;
+  int Ns = 1;
+#pragma unroll
+  for (int p = 0; p < P8; ++p) {
+    if (p > 0) {
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) v[g][m] = buf[g * gs + lpad(t + m * Q)];
+      }
+    }
+    if (Ns > 1) {
+      const int jm = t % Ns;
+      const cd w1 = twiddle<DIR>(tw, Ns * 8, jm);
+      const cd w2 = twiddle<DIR>(tw, Ns * 8, 2 * jm);
+      const cd w4 = twiddle<DIR>(tw, Ns * 8, 4 * jm);
+      const cd w3 = cmul(w1, w2);
+      const cd w5 = cmul(w1, w4), w6 = cmul(w2, w4), w7 = cmul(w3, w4);
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        v[g][1] = cmul(v[g][1], w1);
+        v[g][2] = cmul(v[g][2], w2);
+        v[g][3] = cmul(v[g][3], w3);
+        v[g][4] = cmul(v[g][4], w4);
+        v[g][5] = cmul(v[g][5], w5);
+        v[g][6] = cmul(v[g][6], w6);
+        v[g][7] = cmul(v[g][7], w7);
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) dft8<DIR>(v[g]);
+    const bool last = (p == P8 - 1) && (REM == 0);
+    if (!last) {
+      __syncthreads();
+      const int base = (t / Ns) * Ns * 8 + (t % Ns);
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) buf[g * gs + lpad(base + r * Ns)] = v[g][r];
+      }
+      __syncthreads();
+    }
+    Ns *= 8;
+  }
+  if (REM != 0 && P8 > 0) {
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) v[g][m] = buf[g * gs + lpad(t + m * Q)];
+    }
+  }
+  if (REM == 1) {  // radix-2 final pass, Ns = L/2, 4 butterflies per lane
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const cd w = twiddle<DIR>(tw, L, t + q * Q);
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        cd a = v[g][q], b = cmul(v[g][q + 4], w);
+        dft2<DIR>(a, b);
+        v[g][q] = a;
+        v[g][q + 4] = b;
+      }
+    }
+  } else if (REM == 2) {  // radix-4 final pass, Ns = L/4, 2 butterflies per lane
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int j = t + q * Q;
+      const cd w1 = twiddle<DIR>(tw, L, j);
+      const cd w2 = twiddle<DIR>(tw, L, 2 * j);
+      const cd w3 = twiddle<DIR>(tw, L, 3 * j);
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        cd a0 = v[g][q], a1 = cmul(v[g][q + 2], w1), a2 = cmul(v[g][q + 4], w2),
+           a3 = cmul(v[g][q + 6], w3);
+        dft4<DIR>(a0, a1, a2, a3);
+        v[g][q] = a0;
+        v[g][q + 2] = a1;
+        v[g][q + 4] = a2;
+        v[g][q + 6] = a3;
+      }
     }
   }
 }

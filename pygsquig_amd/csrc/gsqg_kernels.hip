@@ -30,6 +30,9 @@ struct CKShape {
   static constexpr int RPW = R0 < (N / 2) ? R0 : (N / 2);  // rows per block
   static constexpr int NT = RPW * Q;
   static constexpr int PADL = Padded<N>::value;
+  static constexpr int REGION = RPW * PADL;  // LDS elements per field
+  // Prologue fields transformed together (two blocks per CU fit in LDS).
+  static constexpr int FP = REGION * 16 * 4 <= 80 * 1024 ? 4 : (REGION * 16 * 2 <= 80 * 1024 ? 2 : 1);
 };
 
 template <int N>
@@ -40,6 +43,16 @@ struct RKShape {
   static constexpr int WX = W0 < N ? W0 : N;  // x-rows per block
   static constexpr int NT = WX * Q;
   static constexpr int PADM = Padded<M>::value;
+  static constexpr int REGION = WX * PADM;  // LDS elements per field tile
+  // Field tiles transformed together: as many as fit in 80 KB of LDS, so
+  // that two blocks still share a CU.
+  static constexpr int FP = REGION * 16 * 4 <= 80 * 1024 ? 4 : (REGION * 16 * 2 <= 80 * 1024 ? 2 : 1);
+};
+
+// Spectral fields a row-kernel job reads.
+template <int JOB>
+struct RKFields {
+  static constexpr int value = JOB == RK_THETA ? 4 : ((JOB == RK_SCALAR || JOB == RK_VEL) ? 2 : 1);
 };
 
 // ------------------------------------------------------- per-mode physics
@@ -333,7 +346,8 @@ __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
   constexpr int Q = S::Q;
   constexpr int PADL = S::PADL;
   constexpr int NT = S::NT;
-  __shared__ cd lds[S::RPW * PADL];
+  constexpr int NFL = (JOB == CK_FWD || JOB == CK_INV) ? 1 : NFields<JOB>::value;
+  __shared__ cd lds[(S::FP < NFL ? S::FP : NFL) * S::REGION];
   __shared__ double red[NT];
 
   const int tid = threadIdx.x;
@@ -567,11 +581,15 @@ __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
     if (tid == 0) a.part[(size_t)a.rate_slot * a.part_stride + blockIdx.x] = red[0];
   }
 
-  // Multipliers (per mode; row 0 in R layout) and inverse x-FFTs.
+  // Multipliers (per mode; row 0 in R layout) and inverse x-FFTs, G fields
+  // at a time in lockstep.
   constexpr int NF = NFields<JOB>::value;
+  constexpr int G = S::FP < NF ? S::FP : NF;
+  constexpr int RG = S::REGION;
 #pragma unroll
-  for (int q = 0; q < NF; ++q) {
-    const bool needK = (JOB != CK_SCALAR) && (JOB == CK_VEL || q >= 2);
+  for (int g0 = 0; g0 < NF; g0 += G) {
+    cd w[G][8];
+    const bool needK = (JOB == CK_VEL) || (JOB == CK_THETA && g0 + G > 2);
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
       const int i = t + m * Q;
@@ -585,25 +603,37 @@ __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
       const double kx = k1d[md.x], ky = k1d[md.y];
       const double K = needK ? inv_frac(kx * kx + ky * ky, ph) : 0.0;
       const bool selfc = row0 && (i == 0 || i == N / 2);
-      v[m] = selfc ? cmk(0.0, 0.0)
-                   : prologue_field<JOB>(q, y[m], kx, ky, K, md.x == N / 2, md.y == N / 2, ph);
+#pragma unroll
+      for (int g = 0; g < G; ++g)
+        w[g][m] = selfc ? cmk(0.0, 0.0)
+                        : prologue_field<JOB>(g0 + g, y[m], kx, ky, K, md.x == N / 2,
+                                              md.y == N / 2, ph);
     }
     if (blk0) {  // row 0: R layout -> packed Z
       __syncthreads();
       if (row0) {
 #pragma unroll
-        for (int m = 0; m < 8; ++m) buf[lpad(t + m * Q)] = v[m];
+        for (int g = 0; g < G; ++g) {
+#pragma unroll
+          for (int m = 0; m < 8; ++m) buf[g * RG + lpad(t + m * Q)] = w[g][m];
+        }
       }
       __syncthreads();
       if (row0) {
 #pragma unroll
-        for (int m = 0; m < 8; ++m) v[m] = z_from_r(buf, t + m * Q, N);
+        for (int g = 0; g < G; ++g) {
+#pragma unroll
+          for (int m = 0; m < 8; ++m) w[g][m] = z_from_r(buf + g * RG, t + m * Q, N);
+        }
       }
     }
-    fft_lanes<N, +1>(v, buf, t, tw);
-    cd* dst = a.F[q] + fo + (size_t)j * N;
+    fft_lanes_multi<N, +1, G>(w, buf, RG, t, tw);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) dst[t + m * Q] = v[m];
+    for (int g = 0; g < G; ++g) {
+      cd* dst = a.F[g0 + g] + fo + (size_t)j * N;
+#pragma unroll
+      for (int m = 0; m < 8; ++m) dst[t + m * Q] = w[g][m];
+    }
   }
 }
 
@@ -618,43 +648,51 @@ __device__ __forceinline__ int rk_tile(int b) {
   return (b & 7) * (T / 8) + (b >> 3);
 }
 
-// Load one mixed field's tile into LDS (coalesced along x), then pre-twist
-// for the half-length complex inverse: lane element k gets
+// Load G mixed fields' tiles into LDS regions (coalesced along x; every
+// global load is issued before the first barrier), then pre-twist for the
+// half-length complex inverse: lane element k of field g gets
 // Z'(k) = (X(k) + conj X(M-k)) + i e^{+2 pi i k/N} (X(k) - conj X(M-k)).
-template <int N>
-__device__ __forceinline__ void rk_load_c2r(const cd* __restrict__ src, int x0, cd* lds, int tid,
-                                            int b, int t, const cd* __restrict__ tw, cd (&v)[8]) {
+// Region g starts at lds + g * REGION.  Ends without a barrier: the next LDS
+// write (inside the transform) is preceded by one.
+template <int N, int G>
+__device__ __forceinline__ void rk_load_c2r(const cd* const (&src)[G], int x0, cd* lds, int tid,
+                                            int b, int t, const cd* __restrict__ tw,
+                                            cd (&v)[G][8]) {
   using S = RKShape<N>;
-  constexpr int M = S::M, WX = S::WX, NT = S::NT, PADM = S::PADM, Q = S::Q;
-  cd tmp[8];
+  constexpr int M = S::M, WX = S::WX, NT = S::NT, PADM = S::PADM, Q = S::Q, RG = S::REGION;
+  // Staged as native 2-vectors: plain values that stay in registers across
+  // the barrier (an array of the aggregate type is copied through scratch).
+  vd2 tmp[G * 8];
 #pragma unroll
-  for (int r = 0; r < 8; ++r) {
-    const int l = tid + r * NT;
+  for (int e = 0; e < G * 8; ++e) {
+    const int l = tid + (e & 7) * NT;
     const int jr = l / WX, bb = l - jr * WX;
-    tmp[r] = src[(size_t)jr * N + x0 + bb];
+    tmp[e] = *reinterpret_cast<const vd2*>(src[e >> 3] + (size_t)jr * N + x0 + bb);
   }
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < 8; ++r) {
-    const int l = tid + r * NT;
+  for (int e = 0; e < G * 8; ++e) {
+    const int l = tid + (e & 7) * NT;
     const int jr = l / WX, bb = l - jr * WX;
-    lds[bb * PADM + lpad(jr)] = tmp[r];
+    *reinterpret_cast<vd2*>(lds + (e >> 3) * RG + bb * PADM + lpad(jr)) = tmp[e];
   }
   __syncthreads();
-  const cd* buf = lds + b * PADM;
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     const int k = t + m * Q;
-    const cd X = buf[lpad(k)];
-    if (k == 0) {
-      v[m] = cmk(X.x + X.y, X.x - X.y);
-    } else {
-      const cd Xp = cconj(buf[lpad(M - k)]);
-      const cd w = cconj(tw[N + k]);
-      v[m] = cadd(cadd(X, Xp), cmul_i(cmul(w, csub(X, Xp))));
+    const cd w = cconj(tw[N + k]);
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const cd* buf = lds + g * RG + b * PADM;
+      const cd X = buf[lpad(k)];
+      if (k == 0) {
+        v[g][m] = cmk(X.x + X.y, X.x - X.y);
+      } else {
+        const cd Xp = cconj(buf[lpad(M - k)]);
+        v[g][m] = cadd(cadd(X, Xp), cmul_i(cmul(w, csub(X, Xp))));
+      }
     }
   }
-  __syncthreads();
 }
 
 // Forward half-length FFT result in v -> post-twist -> store X(k), packed
@@ -691,8 +729,10 @@ __device__ __forceinline__ void rk_store_r2c(cd (&v)[8], cd* dst, int x0, cd* ld
 template <int N, int JOB>
 __global__ void __launch_bounds__(RKShape<N>::NT) rk_kernel(RKArgs a) {
   using S = RKShape<N>;
-  constexpr int M = S::M, Q = S::Q, WX = S::WX, NT = S::NT, PADM = S::PADM;
-  __shared__ cd lds[WX * PADM];
+  constexpr int M = S::M, Q = S::Q, WX = S::WX, NT = S::NT, PADM = S::PADM, RG = S::REGION;
+  constexpr int NF = RKFields<JOB>::value;
+  constexpr int G = S::FP < NF ? S::FP : NF;  // fields per lockstep group
+  __shared__ cd lds[G * RG];
   __shared__ double red[NT];
   const int tid = threadIdx.x;
   const int b = tid / Q;
@@ -736,70 +776,84 @@ __global__ void __launch_bounds__(RKShape<N>::NT) rk_kernel(RKArgs a) {
     return;
   }
   if (JOB == RK_C2R || JOB == RK_VEL) {
-    const int nf = (JOB == RK_C2R) ? 1 : 2;
-    for (int q = 0; q < nf; ++q) {
-      rk_load_c2r<N>(a.F[q] + fo, x0, lds, tid, b, t, tw, v);
-      fft_lanes<M, +1>(v, buf, t, tw);
-      double* dstp = (JOB == RK_C2R) ? a.phys_out : (q == 0 ? a.uout : a.vout);
-      cd* dst = reinterpret_cast<cd*>(dstp + fo + (size_t)x * N);
 #pragma unroll
-      for (int m = 0; m < 8; ++m) dst[t + m * Q] = v[m];
+    for (int g0 = 0; g0 < NF; g0 += G) {
+      const cd* src[G];
+#pragma unroll
+      for (int g = 0; g < G; ++g) src[g] = a.F[g0 + g] + fo;
+      cd w[G][8];
+      rk_load_c2r<N, G>(src, x0, lds, tid, b, t, tw, w);
+      fft_lanes_multi<M, +1, G>(w, buf, RG, t, tw);
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int q = g0 + g;
+        double* dstp = (JOB == RK_C2R) ? a.phys_out : (q == 0 ? a.uout : a.vout);
+        cd* dst = reinterpret_cast<cd*>(dstp + fo + (size_t)x * N);
+#pragma unroll
+        for (int m = 0; m < 8; ++m) dst[t + m * Q] = w[g][m];
+      }
     }
     return;
   }
 
   // J = u * theta_x + v * theta_y on packed real pairs z(k) = (r(2k), r(2k+1)).
-  cd jz[8], hold[8];
-  if (JOB == RK_THETA) {
-    // theta_x
-    rk_load_c2r<N>(a.F[0] + fo, x0, lds, tid, b, t, tw, v);
-    fft_lanes<M, +1>(v, buf, t, tw);
-#pragma unroll
-    for (int m = 0; m < 8; ++m) hold[m] = v[m];
-    // u
-    rk_load_c2r<N>(a.F[2] + fo, x0, lds, tid, b, t, tw, v);
-    fft_lanes<M, +1>(v, buf, t, tw);
-    if (a.uout) {
-      cd* dst = reinterpret_cast<cd*>(a.uout + (size_t)x * N);
-#pragma unroll
-      for (int m = 0; m < 8; ++m) dst[t + m * Q] = v[m];
-    }
-#pragma unroll
-    for (int m = 0; m < 8; ++m) jz[m] = cmk(v[m].x * hold[m].x, v[m].y * hold[m].y);
-    // theta_y
-    rk_load_c2r<N>(a.F[1] + fo, x0, lds, tid, b, t, tw, v);
-    fft_lanes<M, +1>(v, buf, t, tw);
-#pragma unroll
-    for (int m = 0; m < 8; ++m) hold[m] = v[m];
-    // v
-    rk_load_c2r<N>(a.F[3] + fo, x0, lds, tid, b, t, tw, v);
-    fft_lanes<M, +1>(v, buf, t, tw);
-    if (a.vout) {
-      cd* dst = reinterpret_cast<cd*>(a.vout + (size_t)x * N);
-#pragma unroll
-      for (int m = 0; m < 8; ++m) dst[t + m * Q] = v[m];
-    }
-#pragma unroll
-    for (int m = 0; m < 8; ++m)
-      jz[m] = cmk(jz[m].x + v[m].x * hold[m].x, jz[m].y + v[m].y * hold[m].y);
-  } else {  // RK_SCALAR: frozen physical u, v
+  // Theta fields are taken in the order theta_x, u, theta_y, v (F[0], F[2],
+  // F[1], F[3]); scalar fields s_x, s_y against the frozen physical u, v.
+  cd jz[8], hold[8], uu[8], vv[8];
+  if (JOB == RK_SCALAR) {
     const cd* up = reinterpret_cast<const cd*>(a.uphys + (size_t)x * N);
     const cd* vp = reinterpret_cast<const cd*>(a.vphys + (size_t)x * N);
-    cd uu[8], vv[8];
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
       uu[m] = up[t + m * Q];
       vv[m] = vp[t + m * Q];
     }
-    rk_load_c2r<N>(a.F[0] + fo, x0, lds, tid, b, t, tw, v);
-    fft_lanes<M, +1>(v, buf, t, tw);
+  }
 #pragma unroll
-    for (int m = 0; m < 8; ++m) jz[m] = cmk(uu[m].x * v[m].x, uu[m].y * v[m].y);
-    rk_load_c2r<N>(a.F[1] + fo, x0, lds, tid, b, t, tw, v);
-    fft_lanes<M, +1>(v, buf, t, tw);
+  for (int g0 = 0; g0 < NF; g0 += G) {
+    const cd* src[G];
 #pragma unroll
-    for (int m = 0; m < 8; ++m)
-      jz[m] = cmk(jz[m].x + vv[m].x * v[m].x, jz[m].y + vv[m].y * v[m].y);
+    for (int g = 0; g < G; ++g) {
+      const int s = g0 + g;
+      const int q = (JOB == RK_THETA) ? ((s & 1) ? 2 + (s >> 1) : (s >> 1)) : s;
+      src[g] = a.F[q] + fo;
+    }
+    cd w[G][8];
+    rk_load_c2r<N, G>(src, x0, lds, tid, b, t, tw, w);
+    fft_lanes_multi<M, +1, G>(w, buf, RG, t, tw);
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int s = g0 + g;
+      if (JOB == RK_THETA) {
+        if ((s & 1) == 0) {
+#pragma unroll
+          for (int m = 0; m < 8; ++m) hold[m] = w[g][m];
+        } else {
+          double* side = (s == 1) ? a.uout : a.vout;
+          if (side) {
+            cd* dst = reinterpret_cast<cd*>(side + (size_t)x * N);
+#pragma unroll
+            for (int m = 0; m < 8; ++m) dst[t + m * Q] = w[g][m];
+          }
+          if (s == 1) {
+#pragma unroll
+            for (int m = 0; m < 8; ++m)
+              jz[m] = cmk(w[g][m].x * hold[m].x, w[g][m].y * hold[m].y);
+          } else {
+#pragma unroll
+            for (int m = 0; m < 8; ++m)
+              jz[m] = cmk(jz[m].x + w[g][m].x * hold[m].x, jz[m].y + w[g][m].y * hold[m].y);
+          }
+        }
+      } else if (s == 0) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) jz[m] = cmk(uu[m].x * w[g][m].x, uu[m].y * w[g][m].y);
+      } else {
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+          jz[m] = cmk(jz[m].x + vv[m].x * w[g][m].x, jz[m].y + vv[m].y * w[g][m].y);
+      }
+    }
   }
 #pragma unroll
   for (int m = 0; m < 8; ++m) v[m] = jz[m];
