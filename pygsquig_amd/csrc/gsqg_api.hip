@@ -77,15 +77,17 @@ struct gsqg_plan {
   int n_ring = 0;
   double* norms = nullptr;
   int n_norms = 0;
-  double2* row0 = nullptr;  // packed row-0 staging, N per y-slice
   size_t bytes = 0;
   // live kernel timing (gsqg_profile_advance): events around every launch
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
   std::vector<int> ev_class;
   size_t ev_used = 0;
-  // graph cache
+  // graph cache; the scalar step of a captured step body runs as a second
+  // branch (cap2) beside theta's stages 2-4
   hipStream_t cap = nullptr;
+  hipStream_t cap2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipGraphExec_t gexec = nullptr;
   double g_dt = 0.0;
   int g_key_mode = -1;
@@ -106,12 +108,15 @@ static void free_plan(gsqg_plan* p) {
   void* ptrs[] = {p->th,   p->th2,   p->acc,  p->ys,   p->J,     p->F[0], p->F[1], p->F[2],
                   p->F[3], p->s_th,  p->s_acc, p->s_ys, p->s_J,  p->s_F0, p->s_F1, p->uphys,
                   p->vphys, p->k1d,  p->flags, p->tw,   p->ctrl, p->part, p->ring, p->a_val,
-                  p->a_tmp, p->norms, p->row0};
+                  p->a_tmp, p->norms};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (hipEvent_t e : p->ev_pool) (void)hipEventDestroy(e);
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
   if (p->cap) (void)hipStreamDestroy(p->cap);
+  if (p->cap2) (void)hipStreamDestroy(p->cap2);
+  if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+  if (p->ev_join) (void)hipEventDestroy(p->ev_join);
   delete p;
 }
 
@@ -205,7 +210,6 @@ extern "C" int gsqg_plan_create(const gsqg_plan_desc* d, gsqg_plan** out) {
   ALLOC(p->a_tmp, (size_t)(p->n_ring > 0 ? p->n_ring : 1));
   p->n_norms = full_to_half_num_blocks(N) + 2;
   ALLOC(p->norms, (size_t)p->n_norms);
-  ALLOC(p->row0, (size_t)N * (size_t)(p->n_scalars > 0 ? p->n_scalars : 1));
 #undef ALLOC
 
   // Twiddle tables tw[T + k] = exp(-2 pi i k / T), long double on the host.
@@ -232,6 +236,9 @@ extern "C" int gsqg_plan_create(const gsqg_plan_desc* d, gsqg_plan** out) {
   HIPCHK(hipMemset(p->th, 0, h * sizeof(double2)));
   if (p->n_scalars > 0) HIPCHK(hipMemset(p->s_th, 0, h * p->n_scalars * sizeof(double2)));
   HIPCHK(hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&p->cap2, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming));
   HIPCHK(hipDeviceSynchronize());
   *out = p;
   return GSQG_OK;
@@ -314,7 +321,6 @@ static CKArgs ck_theta(gsqg_plan* p, double2* th, int e, int prologue, int rate_
   a.a_val = p->a_val;
   a.n_a = p->n_ring;
   a.field_stride = 0;
-  a.row0 = p->row0;
   return a;
 }
 
@@ -371,7 +377,6 @@ static int launch_scalar_ck(gsqg_plan* p, int e, int prologue, double dt, hipStr
     a.F[1] = p->s_F1 + off;
     a.F[2] = a.F[3] = nullptr;
     a.ph.kappa = p->kappa[uni ? 0 : g];
-    a.row0 = p->row0 + (uni ? 0 : (size_t)g * p->N);
     HIPCHK(ck(p, CK_SCALAR, a, uni ? p->n_scalars : 1, st));
   }
   return GSQG_OK;
@@ -422,13 +427,27 @@ static int theta_rest(gsqg_plan* p, double dt, bool derived, bool last_prologue,
 }
 
 // Steady-state step body: theta step s (s >= 1) and, if scalars exist, the
-// scalar step s-1 slotted after theta's stage-1 row pass.
+// scalar step s-1 slotted after theta's stage-1 row pass.  With a `side`
+// stream the scalar step forks there after that row pass (which writes the
+// velocity it reads) and joins `st` at the end: the two branches share no
+// buffer -- the scalar kernels read uphys/vphys, the scalar arrays and the
+// constant tables; theta's stages 2-4 write neither the velocity nor any
+// scalar array -- so they may run concurrently.  The next step's stage-1 row
+// pass, which rewrites the velocity, is ordered after the join.
 static int step_body(gsqg_plan* p, double dt, bool derived, bool scalars_pending,
                      bool last_prologue, int last_rate, bool scalar_next_prologue,
-                     hipStream_t st) {
-  const int N = p->N;
+                     hipStream_t st, hipStream_t side = nullptr) {
   HIPCHK(rk(p, RK_THETA, rk_theta(p, 1, derived ? 1 : 0, scalars_pending), 1, st));
   int rc;
+  if (scalars_pending && side) {
+    HIPCHK(hipEventRecord(p->ev_fork, st));
+    HIPCHK(hipStreamWaitEvent(side, p->ev_fork, 0));
+    if ((rc = scalar_step(p, dt, scalar_next_prologue, side))) return rc;
+    HIPCHK(hipEventRecord(p->ev_join, side));
+    if ((rc = theta_rest(p, dt, derived, last_prologue, last_rate, st))) return rc;
+    HIPCHK(hipStreamWaitEvent(st, p->ev_join, 0));
+    return GSQG_OK;
+  }
   if (scalars_pending) {
     if ((rc = scalar_step(p, dt, scalar_next_prologue, st))) return rc;
   }
@@ -468,7 +487,7 @@ extern "C" int gsqg_advance(gsqg_plan* p, double dt, int n_steps, int key_mode, 
         }
         hipGraph_t g;
         HIPCHK(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
-        rc = step_body(p, dt, derived, sc, true, 1, true, p->cap);
+        rc = step_body(p, dt, derived, sc, true, 1, true, p->cap, p->cap2);
         hipError_t ce = hipStreamEndCapture(p->cap, &g);
         if (rc) return rc;
         if (ce != hipSuccess) return fail(GSQG_ERR_HIP, std::string("capture: ") + hipGetErrorString(ce));

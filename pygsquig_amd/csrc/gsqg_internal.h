@@ -77,7 +77,6 @@ struct CKArgs {
   double2* a_val;
   int n_a;
   long long field_stride; // elements between consecutive scalars (blockIdx.y)
-  double2* row0;          // N complex per y-slice: staged packed row-0 state
 };
 
 struct RKArgs {
