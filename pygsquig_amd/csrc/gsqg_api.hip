@@ -22,6 +22,8 @@ hipError_t launch_a_gather(const double2* X, const int2* modes, double2* aval, i
                            double* norm, hipStream_t s);
 hipError_t launch_a_scatter(double2* X, const int2* modes, const double2* aval, int n, int N,
                             hipStream_t s);
+hipError_t launch_a_tab(const int2* modes, const double* k1d, const uint8_t* flags, ATab* out, int n,
+                        int N, hipStream_t s);
 hipError_t launch_a_rhs(const Phys& ph, const double* k1d, const uint8_t* flags, const Ctrl* ctrl,
                         const int2* modes, const double2* aval, double2* out, int n, int N,
                         hipStream_t s);
@@ -72,6 +74,7 @@ struct gsqg_plan {
   double* part = nullptr;
   int part_stride = 0;
   int2* ring = nullptr;
+  ATab* a_tab = nullptr;
   double2* a_val = nullptr;
   double2* a_tmp = nullptr;
   int n_ring = 0;
@@ -107,7 +110,7 @@ static void free_plan(gsqg_plan* p) {
   if (!p) return;
   void* ptrs[] = {p->th,   p->th2,   p->acc,  p->ys,   p->J,     p->F[0], p->F[1], p->F[2],
                   p->F[3], p->s_th,  p->s_acc, p->s_ys, p->s_J,  p->s_F0, p->s_F1, p->uphys,
-                  p->vphys, p->k1d,  p->flags, p->tw,   p->ctrl, p->part, p->ring, p->a_val,
+                  p->vphys, p->k1d,  p->flags, p->tw,   p->ctrl, p->part, p->ring, p->a_tab, p->a_val,
                   p->a_tmp, p->norms};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -206,6 +209,7 @@ extern "C" int gsqg_plan_create(const gsqg_plan_desc* d, gsqg_plan** out) {
   ALLOC(p->part, (size_t)8 * p->part_stride);
   p->n_ring = d->forcing ? d->n_ring : 0;
   ALLOC(p->ring, (size_t)(p->n_ring > 0 ? p->n_ring : 1));
+  ALLOC(p->a_tab, (size_t)(p->n_ring > 0 ? p->n_ring : 1));
   ALLOC(p->a_val, (size_t)(p->n_ring > 0 ? p->n_ring : 1));
   ALLOC(p->a_tmp, (size_t)(p->n_ring > 0 ? p->n_ring : 1));
   p->n_norms = full_to_half_num_blocks(N) + 2;
@@ -229,6 +233,7 @@ extern "C" int gsqg_plan_create(const gsqg_plan_desc* d, gsqg_plan** out) {
     HIPCHK(hipMemcpy(p->ring, d->ring_modes, (size_t)p->n_ring * sizeof(int2),
                      hipMemcpyHostToDevice));
     HIPCHK(hipMemset(p->a_val, 0, (size_t)p->n_ring * sizeof(double2)));
+    HIPCHK(launch_a_tab(p->ring, p->k1d, p->flags, p->a_tab, p->n_ring, N, nullptr));
   }
   Ctrl c0{};
   for (int i = 0; i < 8; ++i) c0.scale[i] = 1.0;
@@ -317,7 +322,7 @@ static CKArgs ck_theta(gsqg_plan* p, double2* th, int e, int prologue, int rate_
   a.rate_next_key = rate_next;
   a.dt = dt;
   a.ph = p->ph;
-  a.a_modes = p->ring;
+  a.a_tab = p->a_tab;
   a.a_val = p->a_val;
   a.n_a = p->n_ring;
   a.field_stride = 0;

@@ -122,93 +122,93 @@ __device__ __forceinline__ cd twiddle(const cd* __restrict__ tw, int T, int k) {
   return DIR < 0 ? w : cconj(w);
 }
 
-// One length-L transform over L/8 lanes (lane index t in [0, L/8)); `buf` is
-// this transform's LDS region of Padded<L> elements.  All lanes of the block
-// must call it together (it contains block barriers).
-template <int L, int DIR>
-__device__ __forceinline__ void fft_lanes(cd (&v)[8], cd* buf, int t, const cd* __restrict__ tw) {
-  static_assert(L >= 8 && (L & (L - 1)) == 0, "power of two >= 8");
-  constexpr int Q = L / 8;
-  constexpr int LOG = __builtin_ctz(L);
-  constexpr int P8 = LOG / 3;
-  constexpr int REM = LOG % 3;
-  int Ns = 1;
-#pragma unroll
-  for (int p = 0; p < P8; ++p) {
-    if (p > 0) {
-#pragma unroll
-      for (int m = 0; m < 8; ++m) v[m] = buf[lpad(t + m * Q)];
-    }
-    if (Ns > 1) {
-      // w^r for r = 1..7 from three table reads (w, w^2, w^4) and four
-      // products; keeps the live twiddle registers low.
-      const int jm = t % Ns;
-      const cd w1 = twiddle<DIR>(tw, Ns * 8, jm);
-      const cd w2 = twiddle<DIR>(tw, Ns * 8, 2 * jm);
-      const cd w4 = twiddle<DIR>(tw, Ns * 8, 4 * jm);
-      const cd w3 = cmul(w1, w2);
-      v[1] = cmul(v[1], w1);
-      v[2] = cmul(v[2], w2);
-      v[3] = cmul(v[3], w3);
-      v[4] = cmul(v[4], w4);
-      v[5] = cmul(v[5], cmul(w1, w4));
-      v[6] = cmul(v[6], cmul(w2, w4));
-      v[7] = cmul(v[7], cmul(w3, w4));
-    }
-    dft8<DIR>(v);
-    bool last = (p == P8 - 1) && (REM == 0);
-    if (!last) {
-      __syncthreads();
-      int base = (t / Ns) * Ns * 8 + (t % Ns);
-#pragma unroll
-      for (int r = 0; r < 8; ++r) buf[lpad(base + r * Ns)] = v[r];
-      __syncthreads();
-    }
-    Ns *= 8;
+// Twiddle sources.  Radix-8 pass p >= 1 (Ns = 8^p) needs w, w^2, w^4 with
+// w = exp(DIR 2 pi i jm / (8 Ns)), jm = t mod Ns; the final radix-2 (radix-4)
+// pass needs exp(DIR 2 pi i r j / L) for r = 1 (r = 1, 2, 3).  `base` is the
+// offset of the pass in the compact layout (ignored by the global table).
+struct TwGlobal {
+  const cd* __restrict__ tw;  // tw[T + k] = exp(-2 pi i k / T)
+  template <int DIR>
+  __device__ __forceinline__ void pass(int, int Ns, int jm, cd& w1, cd& w2, cd& w4) const {
+    w1 = twiddle<DIR>(tw, Ns * 8, jm);
+    w2 = twiddle<DIR>(tw, Ns * 8, 2 * jm);
+    w4 = twiddle<DIR>(tw, Ns * 8, 4 * jm);
   }
-  if (REM == 1) {  // radix-2 final pass, Ns = L/2, 4 butterflies per lane
-    if (P8 > 0) {
-#pragma unroll
-      for (int m = 0; m < 8; ++m) v[m] = buf[lpad(t + m * Q)];
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      int j = t + q * Q;
-      cd a = v[q], b = v[q + 4];
-      b = cmul(b, twiddle<DIR>(tw, L, j));
-      dft2<DIR>(a, b);
-      v[q] = a;
-      v[q + 4] = b;
-    }
-  } else if (REM == 2) {  // radix-4 final pass, Ns = L/4, 2 butterflies per lane
-    if (P8 > 0) {
-#pragma unroll
-      for (int m = 0; m < 8; ++m) v[m] = buf[lpad(t + m * Q)];
-    }
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      int j = t + q * Q;
-      cd a0 = v[q], a1 = v[q + 2], a2 = v[q + 4], a3 = v[q + 6];
-      a1 = cmul(a1, twiddle<DIR>(tw, L, j));
-      a2 = cmul(a2, twiddle<DIR>(tw, L, 2 * j));
-      a3 = cmul(a3, twiddle<DIR>(tw, L, 3 * j));
-      dft4<DIR>(a0, a1, a2, a3);
-      v[q] = a0;
-      v[q + 2] = a1;
-      v[q + 4] = a2;
-      v[q + 6] = a3;
-    }
+  template <int DIR>
+  __device__ __forceinline__ cd fin2(int, int L, int j) const {
+    return twiddle<DIR>(tw, L, j);
   }
-}
+  template <int DIR>
+  __device__ __forceinline__ void fin4(int, int L, int j, cd& w1, cd& w2, cd& w3) const {
+    w1 = twiddle<DIR>(tw, L, j);
+    w2 = twiddle<DIR>(tw, L, 2 * j);
+    w3 = twiddle<DIR>(tw, L, 3 * j);
+  }
+};
 
-// G independent length-L transforms advanced in lockstep: transform g keeps
-// its elements in v[g] (same ownership as fft_lanes) and exchanges through
-// the LDS region buf + g * gs.  The G transforms share one set of barriers
-// and one set of twiddle reads, so a wave has G times the independent work
-// between barriers.
-template <int L, int DIR, int G>
-__device__ __forceinline__ void fft_lanes_multi(cd (&v)[G][8], cd* buf, int gs, int t,
-                                                const cd* __restrict__ tw) {
+// Exact twiddles from an octant table oct[r] = W^r, r = 0 .. LT/8, with
+// W = exp(-2 pi i / LT), resident in LDS (reads wait on lgkmcnt only, so they
+// never wait behind the block's outstanding global stores).  W^k for any
+// 0 <= k < LT follows exactly from the quarter turn W^(LT/4) = -i and
+// W^(LT/4 - r) = -i conj(W^r): every value equals the full table's entry.
+template <int LT>
+struct OctTab {
+  static constexpr int SIZE = LT / 8 + 1;
+  const cd* t;
+  __device__ __forceinline__ cd fwd(int k) const {
+    constexpr int QL = LT / 4;
+    const int q = k / QL;
+    const int r = k - q * QL;
+    const bool lo = r <= LT / 8;
+    const cd c = t[lo ? r : QL - r];
+    const cd w = lo ? c : cmk(-c.y, -c.x);
+    if (q == 0) return w;
+    if (q == 1) return cmk(w.y, -w.x);
+    if (q == 2) return cmk(-w.x, -w.y);
+    return cmk(-w.y, w.x);
+  }
+  __device__ __forceinline__ static void fill(cd* dst, const cd* __restrict__ tw, int tid, int nt) {
+    for (int e = tid; e < SIZE; e += nt) dst[e] = tw[LT + e];
+  }
+};
+
+// Twiddles of a length-L transform from the octant table of LT (L | LT):
+// W_L^k = W_LT^(k LT/L).
+template <int L, int LT>
+struct TwOct {
+  OctTab<LT> o;
+  template <int DIR>
+  __device__ __forceinline__ static cd dir(cd w) {
+    return DIR < 0 ? w : cconj(w);
+  }
+  template <int DIR>
+  __device__ __forceinline__ void pass(int, int Ns, int jm, cd& w1, cd& w2, cd& w4) const {
+    const int s = LT / (8 * Ns);
+    w1 = dir<DIR>(o.fwd(jm * s));
+    w2 = dir<DIR>(o.fwd(2 * jm * s));
+    w4 = dir<DIR>(o.fwd(4 * jm * s));
+  }
+  template <int DIR>
+  __device__ __forceinline__ cd fin2(int, int, int j) const {
+    return dir<DIR>(o.fwd(j * (LT / L)));
+  }
+  template <int DIR>
+  __device__ __forceinline__ void fin4(int, int, int j, cd& w1, cd& w2, cd& w3) const {
+    constexpr int s = LT / L;
+    w1 = dir<DIR>(o.fwd(j * s));
+    w2 = dir<DIR>(o.fwd(2 * j * s));
+    w3 = dir<DIR>(o.fwd(3 * j * s));
+  }
+};
+
+// G independent length-L transforms over L/8 lanes each (lane index t in
+// [0, L/8)), advanced in lockstep: transform g keeps its elements in v[g]
+// (lane t holds elements t + m*L/8, m = 0..7, natural order on entry and
+// exit) and exchanges through the LDS region buf + g * gs.  The G transforms
+// share one set of barriers and one set of twiddle reads.  All lanes of the
+// block must call it together (it contains block barriers).
+template <int L, int DIR, int G, class TW>
+__device__ __forceinline__ void fft_core(cd (&v)[G][8], cd* buf, int gs, int t, const TW& tws) {
   static_assert(L >= 8 && (L & (L - 1)) == 0, "power of two >= 8");
   constexpr int Q = L / 8;
   constexpr int LOG = __builtin_ctz(L);
@@ -226,9 +226,8 @@ __device__ __forceinline__ void fft_lanes_multi(cd (&v)[G][8], cd* buf, int gs, 
     }
     if (Ns > 1) {
       const int jm = t % Ns;
-      const cd w1 = twiddle<DIR>(tw, Ns * 8, jm);
-      const cd w2 = twiddle<DIR>(tw, Ns * 8, 2 * jm);
-      const cd w4 = twiddle<DIR>(tw, Ns * 8, 4 * jm);
+      cd w1, w2, w4;
+      tws.template pass<DIR>(0, Ns, jm, w1, w2, w4);
       const cd w3 = cmul(w1, w2);
       const cd w5 = cmul(w1, w4), w6 = cmul(w2, w4), w7 = cmul(w3, w4);
 #pragma unroll
@@ -267,7 +266,7 @@ __device__ __forceinline__ void fft_lanes_multi(cd (&v)[G][8], cd* buf, int gs, 
   if (REM == 1) {  // radix-2 final pass, Ns = L/2, 4 butterflies per lane
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const cd w = twiddle<DIR>(tw, L, t + q * Q);
+      const cd w = tws.template fin2<DIR>(0, L, t + q * Q);
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         cd a = v[g][q], b = cmul(v[g][q + 4], w);
@@ -279,10 +278,8 @@ __device__ __forceinline__ void fft_lanes_multi(cd (&v)[G][8], cd* buf, int gs, 
   } else if (REM == 2) {  // radix-4 final pass, Ns = L/4, 2 butterflies per lane
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-      const int j = t + q * Q;
-      const cd w1 = twiddle<DIR>(tw, L, j);
-      const cd w2 = twiddle<DIR>(tw, L, 2 * j);
-      const cd w3 = twiddle<DIR>(tw, L, 3 * j);
+      cd w1, w2, w3;
+      tws.template fin4<DIR>(0, L, t + q * Q, w1, w2, w3);
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         cd a0 = v[g][q], a1 = cmul(v[g][q + 2], w1), a2 = cmul(v[g][q + 4], w2),
@@ -295,6 +292,13 @@ __device__ __forceinline__ void fft_lanes_multi(cd (&v)[G][8], cd* buf, int gs, 
       }
     }
   }
+}
+
+// One transform with twiddles from the global table; `buf` is its LDS
+// region of Padded<L> elements.
+template <int L, int DIR>
+__device__ __forceinline__ void fft_lanes(cd (&v)[8], cd* buf, int t, const cd* __restrict__ tw) {
+  fft_core<L, DIR, 1>(*reinterpret_cast<cd(*)[1][8]>(&v), buf, 0, t, TwGlobal{tw});
 }
 
 }  // namespace gsqg

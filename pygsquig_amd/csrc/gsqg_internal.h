@@ -52,6 +52,16 @@ struct Phys {
   double kappa;     // scalar diffusivity
 };
 
+// One mode of the anti-Hermitian ring side state with its wavenumbers and
+// flags gathered once at plan creation, so the column kernel reads it in a
+// single load issued before any of its stores.
+struct ATab {
+  double kx, ky;
+  int ii, jy;  // kx index, ky index (<= N/2)
+  unsigned fl;
+  int dc;
+};
+
 enum CKJob { CK_THETA = 0, CK_SCALAR = 1, CK_VEL = 2, CK_FWD = 3, CK_INV = 4 };
 enum RKJob { RK_THETA = 0, RK_SCALAR = 1, RK_VEL = 2, RK_R2C = 3, RK_C2R = 4 };
 
@@ -73,7 +83,7 @@ struct CKArgs {
   int rate_next_key;     // prologue rate uses the key of step+1
   double dt;
   Phys ph;
-  const int2* a_modes;   // anti-Hermitian ring side state: (i, jy) of each entry
+  const ATab* a_tab;     // anti-Hermitian ring side state: mode of each entry
   double2* a_val;
   int n_a;
   long long field_stride; // elements between consecutive scalars (blockIdx.y)

@@ -31,8 +31,10 @@ struct CKShape {
   static constexpr int NT = RPW * Q;
   static constexpr int PADL = Padded<N>::value;
   static constexpr int REGION = RPW * PADL;  // LDS elements per field
-  // Prologue fields transformed together (two blocks per CU fit in LDS).
-  static constexpr int FP = REGION * 16 * 4 <= 80 * 1024 ? 4 : (REGION * 16 * 2 <= 80 * 1024 ? 2 : 1);
+  // Prologue fields transformed together: two where they fit in 80 KB of
+  // LDS.  Not four: the smaller LDS and register footprint lets theta and
+  // scalar blocks share a CU when the two run as concurrent graph branches.
+  static constexpr int FP = REGION * 16 * 2 <= 80 * 1024 ? 2 : 1;
 };
 
 template <int N>
@@ -44,9 +46,9 @@ struct RKShape {
   static constexpr int NT = WX * Q;
   static constexpr int PADM = Padded<M>::value;
   static constexpr int REGION = WX * PADM;  // LDS elements per field tile
-  // Field tiles transformed together: as many as fit in 80 KB of LDS, so
-  // that two blocks still share a CU.
-  static constexpr int FP = REGION * 16 * 4 <= 80 * 1024 ? 4 : (REGION * 16 * 2 <= 80 * 1024 ? 2 : 1);
+  // Field tiles transformed together (two where they fit in 80 KB of LDS;
+  // see CKShape::FP).
+  static constexpr int FP = REGION * 16 * 2 <= 80 * 1024 ? 2 : 1;
 };
 
 // Spectral fields a row-kernel job reads.
@@ -128,13 +130,6 @@ __device__ __forceinline__ void ring_parts(uint64_t key, int i, int jy, int N, d
 __device__ __forceinline__ uint64_t ctrl_key(const Ctrl* c, int64_t step) {
   return c->key_mode ? step_key(c->seed, step) : c->key;
 }
-
-// Packed row 0 helpers: Z = X0 + i XN with X0, XN Hermitian along kx.
-__device__ __forceinline__ cd unpack0(cd z, cd zp) { return cscale(cadd(z, cconj(zp)), 0.5); }
-__device__ __forceinline__ cd unpackN(cd z, cd zp) {
-  return cscale(cmul_mi(csub(z, cconj(zp))), 0.5);
-}
-__device__ __forceinline__ cd pack(cd x0, cd xn) { return cadd(x0, cmul_i(xn)); }
 
 // Per-mode epilogue state.
 struct ModeIn {
@@ -292,32 +287,72 @@ __device__ __forceinline__ cd r_from_z(const cd* buf, int r, int N) {
   return cscale(cmul_mi(csub(buf[lpad(q)], cconj(buf[lpad(N - q)]))), 0.5);
 }
 
-// Epilogue at one row-0 element in R layout.
-template <int JOB, bool FORCED>
-__device__ __forceinline__ ModeOut row0_elem_epilogue(int E, int r, const ModeIn& in,
-                                                      const double* k1d, const uint8_t* flags,
-                                                      const Phys& ph, double dt, uint64_t key,
-                                                      double sc) {
-  const int N = ph.N;
-  if (r != 0 && r != N / 2) {
-    const int2 md = r_mode(r, N);
-    const double kx = k1d[md.x], ky = k1d[md.y];
-    const unsigned fl = mode_flags(flags, N, md.x, md.y);
-    cd sF = cmk(0.0, 0.0);
-    if (FORCED && (fl & FLAG_RING)) {
-      cd PF, QF;
-      ring_parts(key, md.x, md.y, N, ph.amp, PF, QF);
-      sF = cscale(PF, sc);
+// Per-lane wavenumbers and flags of the column kernel, loaded at kernel
+// start: kx and the raw flag byte of elements t + m*Q of row j, the row's ky
+// and k1d[N/2] (row 0 holds the ky = N/2 modes too).
+struct RowCtx {
+  double kx[8];
+  unsigned fb[8];
+  double ky, kyn;
+};
+
+// Mode of element m (not a row-0 self-conjugate element): wavenumbers,
+// flags, indices and the Nyquist lines its multipliers vanish on.  Row 0 is
+// in R layout: elements m < 4 are (r, 0), elements m >= 4 are (r - N/2, N/2),
+// so both use kx and the flag byte of element m & 3.
+struct MInfo {
+  double kx, ky;
+  unsigned fl;
+  int ii, jy;
+  bool kxnyq, kynyq;
+};
+template <int N>
+__device__ __forceinline__ MInfo minfo(const RowCtx& c, int m, int t, int j, bool row0) {
+  constexpr int Q = N / 8;
+  MInfo r;
+  if (!row0) {
+    r.kx = c.kx[m];
+    r.ky = c.ky;
+    r.fl = c.fb[m] & 15u;
+    r.ii = t + m * Q;
+    r.jy = j;
+    r.kxnyq = (r.ii == N / 2);
+    r.kynyq = false;
+  } else {
+    const int m4 = m & 3;
+    r.kx = c.kx[m4];
+    r.ii = t + m4 * Q;
+    r.kxnyq = false;
+    if (m < 4) {
+      r.ky = c.ky;
+      r.fl = c.fb[m4] & 15u;
+      r.jy = 0;
+      r.kynyq = false;
+    } else {
+      r.ky = c.kyn;
+      r.fl = c.fb[m4] >> 4;
+      r.jy = N / 2;
+      r.kynyq = true;
     }
-    return mode_epilogue<JOB == CK_SCALAR>(E, in, kx * kx + ky * ky, false, fl, ph, dt, FORCED, sF);
   }
-  // two real self-conjugate modes: (r, 0) in the real parts, (r, N/2) in the imaginary parts
+  return r;
+}
+
+// Epilogue at a self-conjugate row-0 element r (0 or N/2): two real modes,
+// (r, 0) in the real parts and (r, N/2) in the imaginary parts; kxr = k1d[r],
+// fbyte = flag byte of column r of row 0.
+template <int JOB, bool FORCED>
+__device__ __forceinline__ ModeOut row0_selfc_epilogue(int E, int r, const ModeIn& in, double kxr,
+                                                       unsigned fbyte, double ky0, double kyn,
+                                                       const Phys& ph, double dt, uint64_t key,
+                                                       double sc) {
+  const int N = ph.N;
   ModeOut oo[2];
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int jy = h ? N / 2 : 0;
-    const double kx = k1d[r], ky = k1d[jy];
-    const unsigned fl = mode_flags(flags, N, r, jy);
+    const double ky = h ? kyn : ky0;
+    const unsigned fl = h ? (fbyte >> 4) : (fbyte & 15u);
     ModeIn u;
     u.J = cmk(h ? in.J.y : in.J.x, 0.0);
     u.th = cmk(h ? in.th.y : in.th.x, 0.0);
@@ -329,8 +364,8 @@ __device__ __forceinline__ ModeOut row0_elem_epilogue(int E, int r, const ModeIn
       ring_parts(key, r, jy, N, ph.amp, PF, QF);
       sF = cscale(PF, sc);
     }
-    oo[h] = mode_epilogue<JOB == CK_SCALAR>(E, u, kx * kx + ky * ky, (r == 0 && jy == 0), fl, ph,
-                                            dt, FORCED, sF);
+    oo[h] = mode_epilogue<JOB == CK_SCALAR>(E, u, kxr * kxr + ky * ky, (r == 0 && jy == 0), fl,
+                                            ph, dt, FORCED, sF);
   }
   ModeOut o;
   o.th = cmk(oo[0].th.x, oo[1].th.x);
@@ -347,8 +382,11 @@ __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
   constexpr int PADL = S::PADL;
   constexpr int NT = S::NT;
   constexpr int NFL = (JOB == CK_FWD || JOB == CK_INV) ? 1 : NFields<JOB>::value;
+  // Twiddles from an LDS octant table where it is small (N <= 2048).
+  constexpr bool TWL = N <= 2048;
   __shared__ cd lds[(S::FP < NFL ? S::FP : NFL) * S::REGION];
-  __shared__ double red[NT];
+  __shared__ cd twl[TWL ? OctTab<N>::SIZE : 1];
+  double* red = reinterpret_cast<double*>(lds);  // reduction scratch, aliases region 0
 
   const int tid = threadIdx.x;
   const int rl = tid / Q;
@@ -407,6 +445,10 @@ __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
     return;
   }
 
+  // ------------------------------------------------------------ loads
+  // Every global load of the kernel is issued here, before its first store:
+  // loads and stores share one in-order completion counter, so a load issued
+  // after a store could only be waited for together with that store.
   const Ctrl* ctrl = a.ctrl;
   const int64_t step = ctrl->step;
   uint64_t key = 0;
@@ -415,22 +457,56 @@ __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
     key = ctrl_key(ctrl, step);
     if (E > 0) sc = ctrl->scale[E];
   }
-
   cd y[8];
+  cd thv[8], accv[8], ysv[8];
+  const bool need_ys = (ph.scheme == 0) && (E >= 3);
   if (E > 0) {
-    cd thv[8], accv[8], ysv[8];
-    constexpr bool need_acc = (E >= 2);
-    const bool need_ys = (ph.scheme == 0) && (E >= 3);
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = a.J[ro + t + m * Q];
 #pragma unroll
     for (int m = 0; m < 8; ++m) thv[m] = a.th[ro + t + m * Q];
 #pragma unroll
-    for (int m = 0; m < 8; ++m) accv[m] = need_acc ? a.acc[ro + t + m * Q] : cmk(0.0, 0.0);
+    for (int m = 0; m < 8; ++m) accv[m] = (E >= 2) ? a.acc[ro + t + m * Q] : cmk(0.0, 0.0);
 #pragma unroll
     for (int m = 0; m < 8; ++m) ysv[m] = need_ys ? a.ys[ro + t + m * Q] : cmk(0.0, 0.0);
+  } else {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) y[m] = a.th[ro + t + m * Q];
+  }
+  RowCtx ctx;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    ctx.kx[m] = k1d[t + m * Q];
+    ctx.fb[m] = a.flags[(size_t)j * N + t + m * Q];
+  }
+  ctx.ky = k1d[j];
+  ctx.kyn = k1d[N / 2];
+  // Anti-Hermitian ring side state: this thread's first entry.
+  constexpr bool AUPD = (JOB == CK_THETA && E == 4 && FORCED);
+  const int ent0 = blockIdx.x + gridDim.x * tid;
+  ATab at0{};
+  cd A0 = cmk(0.0, 0.0);
+  if (AUPD && ent0 < a.n_a) {
+    const ATab* ap = a.a_tab + ent0;
+    at0.kx = ap->kx;
+    at0.ky = ap->ky;
+    at0.ii = ap->ii;
+    at0.jy = ap->jy;
+    at0.fl = ap->fl;
+    at0.dc = ap->dc;
+    A0 = a.a_val[ent0];
+  }
+  if (TWL) OctTab<N>::fill(twl, tw, tid, NT);
+  __syncthreads();
+  const TwOct<N, N> twS{{twl}};
+  const TwGlobal twG{tw};
 
-    fft_lanes<N, -1>(v, buf, t, tw);
+  // --------------------------------------------------------- epilogue
+  if (E > 0) {
+    if (TWL)
+      fft_core<N, -1, 1>(*reinterpret_cast<cd(*)[1][8]>(&v), buf, 0, t, twS);
+    else
+      fft_core<N, -1, 1>(*reinterpret_cast<cd(*)[1][8]>(&v), buf, 0, t, twG);
 
     if (blk0) {  // row 0: forward-transformed Jacobian, packed Z -> R layout
       __syncthreads();
@@ -445,80 +521,82 @@ __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
       }
     }
 
-    const double kyj = k1d[j];
+    // With every table preloaded, no global load follows these stores, so
+    // they are issued as soon as each mode is done.
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
-      const int i = t + m * Q;
       ModeIn in;
       in.J = v[m];
       in.th = thv[m];
       in.acc = accv[m];
       in.ys = ysv[m];
+      const bool selfc = row0 && t == 0 && (m == 0 || m == 4);
       ModeOut o;
-      if (!row0) {
-        const double kx = k1d[i];
-        const unsigned fl = mode_flags(a.flags, N, i, j);
+      if (!selfc) {
+        const MInfo mi = minfo<N>(ctx, m, t, j, row0);
         cd sF = cmk(0.0, 0.0);
-        if (FORCED && (fl & FLAG_RING)) {
+        if (FORCED && (mi.fl & FLAG_RING)) {
           cd PF, QF;
-          ring_parts(key, i, j, N, ph.amp, PF, QF);
+          ring_parts(key, mi.ii, mi.jy, N, ph.amp, PF, QF);
           sF = cscale(PF, sc);
         }
-        o = mode_epilogue<JOB == CK_SCALAR>(E, in, kx * kx + kyj * kyj, false, fl, ph, a.dt,
-                                            FORCED, sF);
+        o = mode_epilogue<JOB == CK_SCALAR>(E, in, mi.kx * mi.kx + mi.ky * mi.ky, false, mi.fl,
+                                            ph, a.dt, FORCED, sF);
       } else {
-        o = row0_elem_epilogue<JOB, FORCED>(E, i, in, k1d, a.flags, ph, a.dt, key, sc);
+        o = row0_selfc_epilogue<JOB, FORCED>(E, m * Q, in, ctx.kx[m], ctx.fb[m], ctx.ky, ctx.kyn,
+                                             ph, a.dt, key, sc);
       }
+      const int i = t + m * Q;
       if (E <= 3) a.acc[ro + i] = o.acc;
       if ((E == 2 || E == 3) && ph.scheme == 0) a.ys[ro + i] = o.ys;
       if (E == 4) a.th[ro + i] = o.th;
       y[m] = o.y;
     }
-    if (E == 4 && JOB == CK_THETA) {
-      bool bad = false;
-#pragma unroll
-      for (int m = 0; m < 8; ++m) bad |= !(isfinite(y[m].x) && isfinite(y[m].y));
-      if (bad) a.ctrl->nonfinite = 1.0;
-    }
-
     // Anti-Hermitian ring side state (final stage, theta only).
-    if (JOB == CK_THETA && E == 4 && FORCED) {
+    if (AUPD) {
       const double s1 = ctrl->scale[1], s2 = ctrl->scale[2], s3 = ctrl->scale[3], s4 = ctrl->scale[4];
-      for (int ent = blockIdx.x + gridDim.x * tid; ent < a.n_a; ent += gridDim.x * NT) {
-        const int2 md = a.a_modes[ent];
-        const int ii = md.x, jy = md.y;
-        const double kx = k1d[ii], ky = k1d[jy];
-        const double k2 = kx * kx + ky * ky;
-        const unsigned fl = mode_flags(a.flags, N, ii, jy);
-        const bool dc = (ii == 0) && (jy == 0);
+      for (int ent = ent0; ent < a.n_a; ent += gridDim.x * NT) {
+        ATab at = at0;
+        cd A = A0;
+        if (ent != ent0) {  // entries beyond one per thread (never at the shipped sizes)
+          const ATab* ap = a.a_tab + ent;
+          at.kx = ap->kx;
+          at.ky = ap->ky;
+          at.ii = ap->ii;
+          at.jy = ap->jy;
+          at.fl = ap->fl;
+          at.dc = ap->dc;
+          A = a.a_val[ent];
+        }
+        const double k2 = at.kx * at.kx + at.ky * at.ky;
+        const bool dc = at.dc != 0;
         cd PF, QF;
-        ring_parts(key, ii, jy, N, ph.amp, PF, QF);
-        const cd A = a.a_val[ent];
+        ring_parts(key, at.ii, at.jy, N, ph.amp, PF, QF);
         const cd zero = cmk(0.0, 0.0);
         cd An;
         if (ph.scheme == 0) {
           const double hdt = 0.5 * a.dt;
-          cd k1 = explicit_rhs(zero, A, k2, dc, fl, ph, true, cscale(QF, s1));
+          cd k1 = explicit_rhs(zero, A, k2, dc, at.fl, ph, true, cscale(QF, s1));
           cd acc = k1;
           cd y2 = cadd(A, cscale(k1, hdt));
-          cd k2v = explicit_rhs(zero, y2, k2, dc, fl, ph, true, cscale(QF, s2));
+          cd k2v = explicit_rhs(zero, y2, k2, dc, at.fl, ph, true, cscale(QF, s2));
           acc = cadd(acc, cscale(k2v, 2.0));
           cd y3 = cadd(A, cscale(k2v, hdt));
-          cd k3 = explicit_rhs(zero, y3, k2, dc, fl, ph, true, cscale(QF, s3));
+          cd k3 = explicit_rhs(zero, y3, k2, dc, at.fl, ph, true, cscale(QF, s3));
           acc = cadd(acc, cscale(k3, 2.0));
           cd y4 = cadd(A, cscale(k3, a.dt));
-          cd k4v = explicit_rhs(zero, y4, k2, dc, fl, ph, true, cscale(QF, s4));
-          cd s = cadd(acc, k4v);
+          cd k4v = explicit_rhs(zero, y4, k2, dc, at.fl, ph, true, cscale(QF, s4));
+          cd sm = cadd(acc, k4v);
           cd d;
           if (ph.combine == 0) {
-            cd tt = cscale(s, a.dt);
+            cd tt = cscale(sm, a.dt);
             d = cmk(tt.x / 6.0, tt.y / 6.0);
           } else {
-            d = cscale(s, a.dt / 6.0);
+            d = cscale(sm, a.dt / 6.0);
           }
           An = cadd(A, d);
         } else {
-          const double L = lin_coeff(k2, dc, fl, ph);
+          const double L = lin_coeff(k2, dc, at.fl, ph);
           const double e1 = exp(L * 0.5 * a.dt), e2 = exp(L * a.dt);
           cd acc = cscale(cscale(QF, s1), e2);
           acc = cadd(acc, cscale(cscale(cscale(QF, s2), e1), 2.0));
@@ -528,9 +606,12 @@ __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
         a.a_val[ent] = An;
       }
     }
-  } else {
+    if (E == 4 && JOB == CK_THETA) {
+      bool bad = false;
 #pragma unroll
-    for (int m = 0; m < 8; ++m) y[m] = a.th[ro + t + m * Q];
+      for (int m = 0; m < 8; ++m) bad |= !(isfinite(y[m].x) && isfinite(y[m].y));
+      if (bad) a.ctrl->nonfinite = 1.0;
+    }
   }
 
   // -------------------------------------------------------------- prologue
@@ -543,35 +624,28 @@ __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
     double accr = 0.0;
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
-      const int i = t + m * Q;
-      if (!row0) {
-        const unsigned fl = mode_flags(a.flags, N, i, j);
-        if (fl & FLAG_RING) {
+      const bool selfc = row0 && t == 0 && (m == 0 || m == 4);
+      if (!selfc) {
+        const MInfo mi = minfo<N>(ctx, m, t, j, row0);
+        if (mi.fl & FLAG_RING) {
           cd PF, QF;
-          ring_parts(rkey, i, j, N, ph.amp, PF, QF);
-          accr += 2.0 * (y[m].x * PF.x + y[m].y * PF.y);
-        }
-      } else if (i != 0 && i != N / 2) {
-        const int2 md = r_mode(i, N);
-        const unsigned fl = mode_flags(a.flags, N, md.x, md.y);
-        if (fl & FLAG_RING) {
-          cd PF, QF;
-          ring_parts(rkey, md.x, md.y, N, ph.amp, PF, QF);
+          ring_parts(rkey, mi.ii, mi.jy, N, ph.amp, PF, QF);
           accr += 2.0 * (y[m].x * PF.x + y[m].y * PF.y);
         }
       } else {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           const int jy = h ? N / 2 : 0;
-          const unsigned fl = mode_flags(a.flags, N, i, jy);
+          const unsigned fl = h ? (ctx.fb[m] >> 4) : (ctx.fb[m] & 15u);
           if (fl & FLAG_RING) {
             cd PF, QF;
-            ring_parts(rkey, i, jy, N, ph.amp, PF, QF);
+            ring_parts(rkey, m * Q, jy, N, ph.amp, PF, QF);
             accr += (h ? y[m].y : y[m].x) * PF.x;
           }
         }
       }
     }
+    __syncthreads();  // region 0 (aliased by red) may still be read
     red[tid] = accr;
     __syncthreads();
     for (int s = NT / 2; s > 0; s >>= 1) {
@@ -592,22 +666,14 @@ __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
     const bool needK = (JOB == CK_VEL) || (JOB == CK_THETA && g0 + G > 2);
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
-      const int i = t + m * Q;
-      int2 md;
-      if (!row0) {
-        md.x = i;
-        md.y = j;
-      } else {
-        md = r_mode(i, N);  // the self-conjugate modes give zero fields
-      }
-      const double kx = k1d[md.x], ky = k1d[md.y];
-      const double K = needK ? inv_frac(kx * kx + ky * ky, ph) : 0.0;
-      const bool selfc = row0 && (i == 0 || i == N / 2);
+      const bool selfc = row0 && t == 0 && (m == 0 || m == 4);  // their fields vanish
+      const MInfo mi = minfo<N>(ctx, m, t, j, row0);
+      const double K = needK ? inv_frac(mi.kx * mi.kx + mi.ky * mi.ky, ph) : 0.0;
 #pragma unroll
       for (int g = 0; g < G; ++g)
         w[g][m] = selfc ? cmk(0.0, 0.0)
-                        : prologue_field<JOB>(g0 + g, y[m], kx, ky, K, md.x == N / 2,
-                                              md.y == N / 2, ph);
+                        : prologue_field<JOB>(g0 + g, y[m], mi.kx, mi.ky, K, mi.kxnyq, mi.kynyq,
+                                              ph);
     }
     if (blk0) {  // row 0: R layout -> packed Z
       __syncthreads();
@@ -627,7 +693,10 @@ __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
         }
       }
     }
-    fft_lanes_multi<N, +1, G>(w, buf, RG, t, tw);
+    if (TWL)
+      fft_core<N, +1, G>(w, buf, RG, t, twS);
+    else
+      fft_core<N, +1, G>(w, buf, RG, t, twG);
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       cd* dst = a.F[g0 + g] + fo + (size_t)j * N;
@@ -648,27 +717,44 @@ __device__ __forceinline__ int rk_tile(int b) {
   return (b & 7) * (T / 8) + (b >> 3);
 }
 
-// Load G mixed fields' tiles into LDS regions (coalesced along x; every
-// global load is issued before the first barrier), then pre-twist for the
-// half-length complex inverse: lane element k of field g gets
-// Z'(k) = (X(k) + conj X(M-k)) + i e^{+2 pi i k/N} (X(k) - conj X(M-k)).
-// Region g starts at lds + g * REGION.  Ends without a barrier: the next LDS
-// write (inside the transform) is preceded by one.
+// Issue the global loads of G mixed fields' tiles (WX x-columns by M rows,
+// coalesced along x) into registers.  Staged as native 2-vectors: plain
+// values that stay in registers (an array of the aggregate type is copied
+// through scratch).
 template <int N, int G>
-__device__ __forceinline__ void rk_load_c2r(const cd* const (&src)[G], int x0, cd* lds, int tid,
-                                            int b, int t, const cd* __restrict__ tw,
-                                            cd (&v)[G][8]) {
+__device__ __forceinline__ void rk_issue(const cd* const (&src)[G], int x0, int tid,
+                                         vd2 (&tmp)[G * 8]) {
   using S = RKShape<N>;
-  constexpr int M = S::M, WX = S::WX, NT = S::NT, PADM = S::PADM, Q = S::Q, RG = S::REGION;
-  // Staged as native 2-vectors: plain values that stay in registers across
-  // the barrier (an array of the aggregate type is copied through scratch).
-  vd2 tmp[G * 8];
+  constexpr int WX = S::WX, NT = S::NT;
 #pragma unroll
   for (int e = 0; e < G * 8; ++e) {
     const int l = tid + (e & 7) * NT;
     const int jr = l / WX, bb = l - jr * WX;
     tmp[e] = *reinterpret_cast<const vd2*>(src[e >> 3] + (size_t)jr * N + x0 + bb);
   }
+}
+
+// Pre/post-twist factor e^{-2 pi i k/N}, k < N/2: from the LDS octant
+// table (TWL) or from the global table.
+template <int N, bool TWL>
+struct PreTw {
+  const cd* t;
+  __device__ __forceinline__ cd operator()(int k) const {
+    if (TWL) return OctTab<N>{t}.fwd(k);
+    return t[N + k];
+  }
+};
+
+// Write staged tiles to the LDS regions (region g at lds + g * REGION), then
+// pre-twist for the half-length complex inverse: lane element k of field g
+// gets Z'(k) = (X(k) + conj X(M-k)) + i e^{+2 pi i k/N} (X(k) - conj X(M-k)),
+// with pre(k) = e^{-2 pi i k/N}.  Ends without a barrier: the next LDS write
+// (inside the transform) is preceded by one.
+template <int N, int G, class PT>
+__device__ __forceinline__ void rk_commit(const vd2 (&tmp)[G * 8], cd* lds, int tid, int b, int t,
+                                          const PT& pre, cd (&v)[G][8]) {
+  using S = RKShape<N>;
+  constexpr int M = S::M, WX = S::WX, NT = S::NT, PADM = S::PADM, Q = S::Q, RG = S::REGION;
   __syncthreads();
 #pragma unroll
   for (int e = 0; e < G * 8; ++e) {
@@ -680,7 +766,7 @@ __device__ __forceinline__ void rk_load_c2r(const cd* const (&src)[G], int x0, c
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     const int k = t + m * Q;
-    const cd w = cconj(tw[N + k]);
+    const cd w = cconj(pre(k));
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       const cd* buf = lds + g * RG + b * PADM;
@@ -697,9 +783,9 @@ __device__ __forceinline__ void rk_load_c2r(const cd* const (&src)[G], int x0, c
 
 // Forward half-length FFT result in v -> post-twist -> store X(k), packed
 // slot 0 = X(0) + i X(M), at dst[k*N + x0 + b] (coalesced along x).
-template <int N>
+template <int N, class PT>
 __device__ __forceinline__ void rk_store_r2c(cd (&v)[8], cd* dst, int x0, cd* lds, int tid, int b,
-                                             int t, const cd* __restrict__ tw) {
+                                             int t, const PT& pre) {
   using S = RKShape<N>;
   constexpr int M = S::M, WX = S::WX, NT = S::NT, PADM = S::PADM, Q = S::Q;
   cd* buf = lds + b * PADM;
@@ -720,7 +806,7 @@ __device__ __forceinline__ void rk_store_r2c(cd (&v)[8], cd* dst, int x0, cd* ld
       const cd Zp = cconj(bf[lpad(M - jr)]);
       const cd E = cscale(cadd(Z, Zp), 0.5);
       const cd O = cscale(cmul_mi(csub(Z, Zp)), 0.5);
-      X = cadd(E, cmul(tw[N + jr], O));
+      X = cadd(E, cmul(pre(jr), O));
     }
     dst[(size_t)jr * N + x0 + bb] = X;
   }
@@ -732,8 +818,12 @@ __global__ void __launch_bounds__(RKShape<N>::NT) rk_kernel(RKArgs a) {
   constexpr int M = S::M, Q = S::Q, WX = S::WX, NT = S::NT, PADM = S::PADM, RG = S::REGION;
   constexpr int NF = RKFields<JOB>::value;
   constexpr int G = S::FP < NF ? S::FP : NF;  // fields per lockstep group
+  // Twiddles and pre/post-twist factors from one LDS octant table of
+  // W_N where it is small (N <= 2048); W_M = W_N^2.
+  constexpr bool TWL = N <= 2048;
   __shared__ cd lds[G * RG];
-  __shared__ double red[NT];
+  __shared__ cd twl[TWL ? OctTab<N>::SIZE : 1];
+  double* red = reinterpret_cast<double*>(lds);  // block 0's reduction, before any tile
   const int tid = threadIdx.x;
   const int b = tid / Q;
   const int t = tid - b * Q;
@@ -766,40 +856,24 @@ __global__ void __launch_bounds__(RKShape<N>::NT) rk_kernel(RKArgs a) {
     a.ctrl->step = a.ctrl->step + 1;
   }
 
+  // Loads first (they share one in-order completion counter with stores):
+  // the first tile group, the frozen velocity, the LDS tables.
+  vd2 tmp[G * 8];
   cd v[8];
+  cd uu[8], vv[8];
   if (JOB == RK_R2C) {
     const cd* src = reinterpret_cast<const cd*>(a.phys_in + fo + (size_t)x * N);
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = src[t + m * Q];
-    fft_lanes<M, -1>(v, buf, t, tw);
-    rk_store_r2c<N>(v, a.Jout + fo, x0, lds, tid, b, t, tw);
-    return;
-  }
-  if (JOB == RK_C2R || JOB == RK_VEL) {
+  } else {
+    const cd* src[G];
 #pragma unroll
-    for (int g0 = 0; g0 < NF; g0 += G) {
-      const cd* src[G];
-#pragma unroll
-      for (int g = 0; g < G; ++g) src[g] = a.F[g0 + g] + fo;
-      cd w[G][8];
-      rk_load_c2r<N, G>(src, x0, lds, tid, b, t, tw, w);
-      fft_lanes_multi<M, +1, G>(w, buf, RG, t, tw);
-#pragma unroll
-      for (int g = 0; g < G; ++g) {
-        const int q = g0 + g;
-        double* dstp = (JOB == RK_C2R) ? a.phys_out : (q == 0 ? a.uout : a.vout);
-        cd* dst = reinterpret_cast<cd*>(dstp + fo + (size_t)x * N);
-#pragma unroll
-        for (int m = 0; m < 8; ++m) dst[t + m * Q] = w[g][m];
-      }
+    for (int g = 0; g < G; ++g) {
+      const int q = (JOB == RK_THETA) ? ((g & 1) ? 2 + (g >> 1) : (g >> 1)) : g;
+      src[g] = a.F[q] + fo;
     }
-    return;
+    rk_issue<N, G>(src, x0, tid, tmp);
   }
-
-  // J = u * theta_x + v * theta_y on packed real pairs z(k) = (r(2k), r(2k+1)).
-  // Theta fields are taken in the order theta_x, u, theta_y, v (F[0], F[2],
-  // F[1], F[3]); scalar fields s_x, s_y against the frozen physical u, v.
-  cd jz[8], hold[8], uu[8], vv[8];
   if (JOB == RK_SCALAR) {
     const cd* up = reinterpret_cast<const cd*>(a.uphys + (size_t)x * N);
     const cd* vp = reinterpret_cast<const cd*>(a.vphys + (size_t)x * N);
@@ -809,22 +883,52 @@ __global__ void __launch_bounds__(RKShape<N>::NT) rk_kernel(RKArgs a) {
       vv[m] = vp[t + m * Q];
     }
   }
+  if (TWL) OctTab<N>::fill(twl, tw, tid, NT);
+  const PreTw<N, TWL> pre{TWL ? twl : tw};
+  const TwOct<M, N> twS{{twl}};
+  const TwGlobal twG{tw};
+
+  if (JOB == RK_R2C) {
+    __syncthreads();
+    if (TWL)
+      fft_core<M, -1, 1>(*reinterpret_cast<cd(*)[1][8]>(&v), buf, 0, t, twS);
+    else
+      fft_core<M, -1, 1>(*reinterpret_cast<cd(*)[1][8]>(&v), buf, 0, t, twG);
+    rk_store_r2c<N>(v, a.Jout + fo, x0, lds, tid, b, t, pre);
+    return;
+  }
+
+  // Theta: fields in the order theta_x, u, theta_y, v (F[0], F[2], F[1],
+  // F[3]), J = u * theta_x + v * theta_y on packed real pairs
+  // z(k) = (r(2k), r(2k+1)); scalars: s_x, s_y against the frozen u, v.
+  cd jz[8], hold[8];
 #pragma unroll
   for (int g0 = 0; g0 < NF; g0 += G) {
-    const cd* src[G];
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-      const int s = g0 + g;
-      const int q = (JOB == RK_THETA) ? ((s & 1) ? 2 + (s >> 1) : (s >> 1)) : s;
-      src[g] = a.F[q] + fo;
-    }
     cd w[G][8];
-    rk_load_c2r<N, G>(src, x0, lds, tid, b, t, tw, w);
-    fft_lanes_multi<M, +1, G>(w, buf, RG, t, tw);
+    rk_commit<N, G>(tmp, lds, tid, b, t, pre, w);
+    if (TWL)
+      fft_core<M, +1, G>(w, buf, RG, t, twS);
+    else
+      fft_core<M, +1, G>(w, buf, RG, t, twG);
+    if (g0 + G < NF) {  // next group's loads, ahead of this group's stores
+      const cd* src[G];
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int s = g0 + G + g;
+        const int q = (JOB == RK_THETA) ? ((s & 1) ? 2 + (s >> 1) : (s >> 1)) : s;
+        src[g] = a.F[q] + fo;
+      }
+      rk_issue<N, G>(src, x0, tid, tmp);
+    }
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       const int s = g0 + g;
-      if (JOB == RK_THETA) {
+      if (JOB == RK_C2R || JOB == RK_VEL) {
+        double* dstp = (JOB == RK_C2R) ? a.phys_out : (s == 0 ? a.uout : a.vout);
+        cd* dst = reinterpret_cast<cd*>(dstp + fo + (size_t)x * N);
+#pragma unroll
+        for (int m = 0; m < 8; ++m) dst[t + m * Q] = w[g][m];
+      } else if (JOB == RK_THETA) {
         if ((s & 1) == 0) {
 #pragma unroll
           for (int m = 0; m < 8; ++m) hold[m] = w[g][m];
@@ -855,11 +959,15 @@ __global__ void __launch_bounds__(RKShape<N>::NT) rk_kernel(RKArgs a) {
       }
     }
   }
+  if (JOB == RK_C2R || JOB == RK_VEL) return;
 #pragma unroll
   for (int m = 0; m < 8; ++m) v[m] = jz[m];
   __syncthreads();
-  fft_lanes<M, -1>(v, buf, t, tw);
-  rk_store_r2c<N>(v, a.Jout + fo, x0, lds, tid, b, t, tw);
+  if (TWL)
+    fft_core<M, -1, 1>(*reinterpret_cast<cd(*)[1][8]>(&v), buf, 0, t, twS);
+  else
+    fft_core<M, -1, 1>(*reinterpret_cast<cd(*)[1][8]>(&v), buf, 0, t, twG);
+  rk_store_r2c<N>(v, a.Jout + fo, x0, lds, tid, b, t, pre);
 }
 
 #if defined(GSQG_INSTANTIATE_COMMON)
@@ -1011,6 +1119,22 @@ __global__ void a_scatter_kernel(cd* X, const int2* modes, const cd* aval, int n
       const size_t om = (size_t)mi * N + mj;
       X[om] = csub(X[om], cconj(A));
     }
+  }
+}
+
+// Side-state table: wavenumbers and flags of every ring mode, gathered once.
+__global__ void a_tab_kernel(const int2* modes, const double* k1d, const uint8_t* flags, ATab* out,
+                             int n, int N) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const int i = modes[e].x, jy = modes[e].y;
+    ATab t;
+    t.kx = k1d[i];
+    t.ky = k1d[jy];
+    t.ii = i;
+    t.jy = jy;
+    t.fl = mode_flags(flags, N, i, jy);
+    t.dc = (i == 0 && jy == 0) ? 1 : 0;
+    out[e] = t;
   }
 }
 
@@ -1209,6 +1333,13 @@ hipError_t launch_a_scatter(cd* X, const int2* modes, const cd* aval, int n, int
   int blocks = (n + 255) / 256;
   if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(a_scatter_kernel, dim3(blocks), dim3(256), 0, s, X, modes, aval, n, N);
+  return hipGetLastError();
+}
+hipError_t launch_a_tab(const int2* modes, const double* k1d, const uint8_t* flags, ATab* out, int n,
+                        int N, hipStream_t s) {
+  int blocks = (n + 255) / 256;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(a_tab_kernel, dim3(blocks), dim3(256), 0, s, modes, k1d, flags, out, n, N);
   return hipGetLastError();
 }
 hipError_t launch_a_rhs(const Phys& ph, const double* k1d, const uint8_t* flags, const Ctrl* ctrl,
