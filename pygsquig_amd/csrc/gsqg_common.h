@@ -128,11 +128,17 @@ __device__ __forceinline__ cd twiddle(const cd* __restrict__ tw, int T, int k) {
 // offset of the pass in the compact layout (ignored by the global table).
 struct TwGlobal {
   const cd* __restrict__ tw;  // tw[T + k] = exp(-2 pi i k / T)
+  // w[r] = W^(r jm), r = 1..7: three table reads (w, w^2, w^4) and four
+  // products, which keeps the live registers low.
   template <int DIR>
-  __device__ __forceinline__ void pass(int, int Ns, int jm, cd& w1, cd& w2, cd& w4) const {
-    w1 = twiddle<DIR>(tw, Ns * 8, jm);
-    w2 = twiddle<DIR>(tw, Ns * 8, 2 * jm);
-    w4 = twiddle<DIR>(tw, Ns * 8, 4 * jm);
+  __device__ __forceinline__ void pass(int Ns, int jm, cd (&w)[8]) const {
+    w[1] = twiddle<DIR>(tw, Ns * 8, jm);
+    w[2] = twiddle<DIR>(tw, Ns * 8, 2 * jm);
+    w[4] = twiddle<DIR>(tw, Ns * 8, 4 * jm);
+    w[3] = cmul(w[1], w[2]);
+    w[5] = cmul(w[1], w[4]);
+    w[6] = cmul(w[2], w[4]);
+    w[7] = cmul(w[3], w[4]);
   }
   template <int DIR>
   __device__ __forceinline__ cd fin2(int, int L, int j) const {
@@ -181,12 +187,18 @@ struct TwOct {
   __device__ __forceinline__ static cd dir(cd w) {
     return DIR < 0 ? w : cconj(w);
   }
+  // w[r] = W^(r jm), r = 1..7: three exact table values (w, w^2, w^4) and
+  // four products, which keeps the live registers low.
   template <int DIR>
-  __device__ __forceinline__ void pass(int, int Ns, int jm, cd& w1, cd& w2, cd& w4) const {
+  __device__ __forceinline__ void pass(int Ns, int jm, cd (&w)[8]) const {
     const int s = LT / (8 * Ns);
-    w1 = dir<DIR>(o.fwd(jm * s));
-    w2 = dir<DIR>(o.fwd(2 * jm * s));
-    w4 = dir<DIR>(o.fwd(4 * jm * s));
+    w[1] = dir<DIR>(o.fwd(jm * s));
+    w[2] = dir<DIR>(o.fwd(2 * jm * s));
+    w[4] = dir<DIR>(o.fwd(4 * jm * s));
+    w[3] = cmul(w[1], w[2]);
+    w[5] = cmul(w[1], w[4]);
+    w[6] = cmul(w[2], w[4]);
+    w[7] = cmul(w[3], w[4]);
   }
   template <int DIR>
   __device__ __forceinline__ cd fin2(int, int, int j) const {
@@ -226,19 +238,12 @@ __device__ __forceinline__ void fft_core(cd (&v)[G][8], cd* buf, int gs, int t, 
     }
     if (Ns > 1) {
       const int jm = t % Ns;
-      cd w1, w2, w4;
-      tws.template pass<DIR>(0, Ns, jm, w1, w2, w4);
-      const cd w3 = cmul(w1, w2);
-      const cd w5 = cmul(w1, w4), w6 = cmul(w2, w4), w7 = cmul(w3, w4);
+      cd w[8];
+      tws.template pass<DIR>(Ns, jm, w);
 #pragma unroll
       for (int g = 0; g < G; ++g) {
-        v[g][1] = cmul(v[g][1], w1);
-        v[g][2] = cmul(v[g][2], w2);
-        v[g][3] = cmul(v[g][3], w3);
-        v[g][4] = cmul(v[g][4], w4);
-        v[g][5] = cmul(v[g][5], w5);
-        v[g][6] = cmul(v[g][6], w6);
-        v[g][7] = cmul(v[g][7], w7);
+#pragma unroll
+        for (int r = 1; r < 8; ++r) v[g][r] = cmul(v[g][r], w[r]);
       }
     }
 #pragma unroll

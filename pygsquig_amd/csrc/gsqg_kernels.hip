@@ -195,7 +195,8 @@ __device__ __forceinline__ ModeOut mode_epilogue(int e, const ModeIn& in, double
     }
   } else {
     double L = SCALAR ? -ph.kappa * k2 : lin_coeff(k2, dc, fl, ph);
-    double e1 = exp(L * hdt), e2 = exp(L * dt);
+    // exp(L dt) = exp(L dt/2)^2: one exponential per mode
+    const double e1 = exp(L * hdt), e2 = e1 * e1;
     cd k = adv;
     if (!SCALAR && forced) k = cadd(k, sF);
     if (e == 1) {
@@ -597,7 +598,7 @@ __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
           An = cadd(A, d);
         } else {
           const double L = lin_coeff(k2, dc, at.fl, ph);
-          const double e1 = exp(L * 0.5 * a.dt), e2 = exp(L * a.dt);
+          const double e1 = exp(L * 0.5 * a.dt), e2 = e1 * e1;
           cd acc = cscale(cscale(QF, s1), e2);
           acc = cadd(acc, cscale(cscale(cscale(QF, s2), e1), 2.0));
           acc = cadd(acc, cscale(cscale(cscale(QF, s3), e1), 2.0));
