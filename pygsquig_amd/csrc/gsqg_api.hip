@@ -11,8 +11,10 @@
 #include "gsqg_internal.h"
 
 namespace gsqg {
-hipError_t launch_ck(int N, int job, const CKArgs& a, int ny, hipStream_t s);
-hipError_t launch_rk(int N, int job, const RKArgs& a, int ny, hipStream_t s);
+hipError_t launch_ck(int N, int job, const CKArgs& a, int ny, hipStream_t s,
+                     const LaunchEv* ev = nullptr);
+hipError_t launch_rk(int N, int job, const RKArgs& a, int ny, hipStream_t s,
+                     const LaunchEv* ev = nullptr);
 int ck_num_blocks(int N);
 hipError_t launch_full_to_half(const double2* X, double2* H, int N, double* norms, double* norm0,
                                hipStream_t s);
@@ -267,38 +269,37 @@ extern "C" int gsqg_plan_device_bytes(const gsqg_plan* p, size_t* bytes) {
 enum { KC_THETA_COL = 0, KC_THETA_ROW = 1, KC_SCALAR_COL = 2, KC_SCALAR_ROW = 3, KC_OTHER = 4,
        KC_COUNT = 5 };
 
-static hipError_t prof_mark(gsqg_plan* p, int cls, hipStream_t st) {
-  if (p->ev_used >= p->ev_pool.size()) {
-    hipEvent_t e;
-    hipError_t r = hipEventCreate(&e);
-    if (r != hipSuccess) return r;
-    p->ev_pool.push_back(e);
-    p->ev_class.push_back(cls);
+// Live timing: a pair of pool events recorded by the launch itself at the
+// kernel's start and end.
+static hipError_t prof_pair(gsqg_plan* p, int cls, LaunchEv* ev) {
+  for (int k = 0; k < 2; ++k) {
+    if (p->ev_used >= p->ev_pool.size()) {
+      hipEvent_t e;
+      hipError_t r = hipEventCreate(&e);
+      if (r != hipSuccess) return r;
+      p->ev_pool.push_back(e);
+      p->ev_class.push_back(-1);
+    }
+    p->ev_class[p->ev_used] = k == 0 ? cls : -1;
+    (k == 0 ? ev->start : ev->stop) = p->ev_pool[p->ev_used++];
   }
-  p->ev_class[p->ev_used] = cls;
-  return hipEventRecord(p->ev_pool[p->ev_used++], st);
+  return hipSuccess;
 }
 
 static hipError_t ck(gsqg_plan* p, int job, const CKArgs& a, int ny, hipStream_t st) {
   const int cls = job == CK_THETA ? KC_THETA_COL : job == CK_SCALAR ? KC_SCALAR_COL : KC_OTHER;
-  if (p->prof) {
-    hipError_t r = prof_mark(p, cls, st);
-    if (r != hipSuccess) return r;
-  }
-  hipError_t r = launch_ck(p->N, job, a, ny, st);
-  if (r == hipSuccess && p->prof) r = prof_mark(p, -1, st);
-  return r;
+  if (!p->prof) return launch_ck(p->N, job, a, ny, st);
+  LaunchEv ev{};
+  hipError_t r = prof_pair(p, cls, &ev);
+  return r != hipSuccess ? r : launch_ck(p->N, job, a, ny, st, &ev);
 }
 
 static hipError_t rk(gsqg_plan* p, int job, const RKArgs& a, int ny, hipStream_t st) {
   const int cls = job == RK_THETA ? KC_THETA_ROW : job == RK_SCALAR ? KC_SCALAR_ROW : KC_OTHER;
-  if (p->prof) {
-    hipError_t r = prof_mark(p, cls, st);
-    if (r != hipSuccess) return r;
-  }
-  hipError_t r = launch_rk(p->N, job, a, ny, st);
-  if (r == hipSuccess && p->prof) r = prof_mark(p, -1, st);
-  return r;
+  if (!p->prof) return launch_rk(p->N, job, a, ny, st);
+  LaunchEv ev{};
+  hipError_t r = prof_pair(p, cls, &ev);
+  return r != hipSuccess ? r : launch_rk(p->N, job, a, ny, st, &ev);
 }
 
 // ----------------------------------------------------------- arg builders

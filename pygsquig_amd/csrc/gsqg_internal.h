@@ -65,6 +65,12 @@ struct ATab {
 enum CKJob { CK_THETA = 0, CK_SCALAR = 1, CK_VEL = 2, CK_FWD = 3, CK_INV = 4 };
 enum RKJob { RK_THETA = 0, RK_SCALAR = 1, RK_VEL = 2, RK_R2C = 3, RK_C2R = 4 };
 
+// Optional start/stop events recorded at the kernel's own boundaries
+// (hipExtLaunchKernelGGL), for live per-kernel timing.
+struct LaunchEv {
+  hipEvent_t start, stop;
+};
+
 struct CKArgs {
   const double2* J;      // mixed input (forward x-transform in the epilogue)
   double2* th;

@@ -17,6 +17,8 @@
 //
 // Both kernels keep each transform in LDS/registers: a field crosses HBM
 // exactly once per read or write.
+#include <hip/hip_ext.h>
+
 #include "gsqg_common.h"
 #include "gsqg_internal.h"
 
@@ -1207,55 +1209,62 @@ __global__ void set_ctrl_kernel(Ctrl* c, int64_t step, uint64_t key, uint64_t se
 // The build compiles this file once per grid size (GSQG_INSTANTIATE_N) and
 // once for the size-independent parts (GSQG_INSTANTIATE_COMMON).
 template <int N, int JOB, int E, bool FORCED>
-static hipError_t launch_ck_n(const CKArgs& a, int nfields_y, hipStream_t s) {
+static hipError_t launch_ck_n(const CKArgs& a, int nfields_y, hipStream_t s, const LaunchEv* ev) {
   using S = CKShape<N>;
   dim3 grid((N / 2) / S::RPW, nfields_y);
-  hipLaunchKernelGGL((ck_kernel<N, JOB, E, FORCED>), grid, dim3(S::NT), 0, s, a);
+  if (ev)
+    hipExtLaunchKernelGGL((ck_kernel<N, JOB, E, FORCED>), grid, dim3(S::NT), 0, s, ev->start,
+                          ev->stop, 0, a);
+  else
+    hipLaunchKernelGGL((ck_kernel<N, JOB, E, FORCED>), grid, dim3(S::NT), 0, s, a);
   return hipGetLastError();
 }
 
 template <int N, int JOB, bool FORCED>
-static hipError_t launch_ck_e(const CKArgs& a, int ny, hipStream_t s) {
+static hipError_t launch_ck_e(const CKArgs& a, int ny, hipStream_t s, const LaunchEv* ev) {
   switch (a.e) {
-    case 0: return launch_ck_n<N, JOB, 0, FORCED>(a, ny, s);
-    case 1: return launch_ck_n<N, JOB, 1, FORCED>(a, ny, s);
-    case 2: return launch_ck_n<N, JOB, 2, FORCED>(a, ny, s);
-    case 3: return launch_ck_n<N, JOB, 3, FORCED>(a, ny, s);
-    case 4: return launch_ck_n<N, JOB, 4, FORCED>(a, ny, s);
+    case 0: return launch_ck_n<N, JOB, 0, FORCED>(a, ny, s, ev);
+    case 1: return launch_ck_n<N, JOB, 1, FORCED>(a, ny, s, ev);
+    case 2: return launch_ck_n<N, JOB, 2, FORCED>(a, ny, s, ev);
+    case 3: return launch_ck_n<N, JOB, 3, FORCED>(a, ny, s, ev);
+    case 4: return launch_ck_n<N, JOB, 4, FORCED>(a, ny, s, ev);
   }
   return hipErrorInvalidValue;
 }
 
 template <int N, int JOB>
-static hipError_t launch_rk_n(const RKArgs& a, int nfields_y, hipStream_t s) {
+static hipError_t launch_rk_n(const RKArgs& a, int nfields_y, hipStream_t s, const LaunchEv* ev) {
   using S = RKShape<N>;
   dim3 grid(N / S::WX, nfields_y);
-  hipLaunchKernelGGL((rk_kernel<N, JOB>), grid, dim3(S::NT), 0, s, a);
+  if (ev)
+    hipExtLaunchKernelGGL((rk_kernel<N, JOB>), grid, dim3(S::NT), 0, s, ev->start, ev->stop, 0, a);
+  else
+    hipLaunchKernelGGL((rk_kernel<N, JOB>), grid, dim3(S::NT), 0, s, a);
   return hipGetLastError();
 }
 
 template <int N>
-hipError_t ck_by_job(int job, const CKArgs& a, int ny, hipStream_t s) {
+hipError_t ck_by_job(int job, const CKArgs& a, int ny, hipStream_t s, const LaunchEv* ev) {
   switch (job) {
     case CK_THETA:
-      return a.ph.forcing ? launch_ck_e<N, CK_THETA, true>(a, ny, s)
-                          : launch_ck_e<N, CK_THETA, false>(a, ny, s);
-    case CK_SCALAR: return launch_ck_e<N, CK_SCALAR, false>(a, ny, s);
-    case CK_VEL: return launch_ck_n<N, CK_VEL, 0, false>(a, ny, s);
-    case CK_FWD: return launch_ck_n<N, CK_FWD, 0, false>(a, ny, s);
-    case CK_INV: return launch_ck_n<N, CK_INV, 0, false>(a, ny, s);
+      return a.ph.forcing ? launch_ck_e<N, CK_THETA, true>(a, ny, s, ev)
+                          : launch_ck_e<N, CK_THETA, false>(a, ny, s, ev);
+    case CK_SCALAR: return launch_ck_e<N, CK_SCALAR, false>(a, ny, s, ev);
+    case CK_VEL: return launch_ck_n<N, CK_VEL, 0, false>(a, ny, s, ev);
+    case CK_FWD: return launch_ck_n<N, CK_FWD, 0, false>(a, ny, s, ev);
+    case CK_INV: return launch_ck_n<N, CK_INV, 0, false>(a, ny, s, ev);
   }
   return hipErrorInvalidValue;
 }
 
 template <int N>
-hipError_t rk_by_job(int job, const RKArgs& a, int ny, hipStream_t s) {
+hipError_t rk_by_job(int job, const RKArgs& a, int ny, hipStream_t s, const LaunchEv* ev) {
   switch (job) {
-    case RK_THETA: return launch_rk_n<N, RK_THETA>(a, ny, s);
-    case RK_SCALAR: return launch_rk_n<N, RK_SCALAR>(a, ny, s);
-    case RK_VEL: return launch_rk_n<N, RK_VEL>(a, ny, s);
-    case RK_R2C: return launch_rk_n<N, RK_R2C>(a, ny, s);
-    case RK_C2R: return launch_rk_n<N, RK_C2R>(a, ny, s);
+    case RK_THETA: return launch_rk_n<N, RK_THETA>(a, ny, s, ev);
+    case RK_SCALAR: return launch_rk_n<N, RK_SCALAR>(a, ny, s, ev);
+    case RK_VEL: return launch_rk_n<N, RK_VEL>(a, ny, s, ev);
+    case RK_R2C: return launch_rk_n<N, RK_R2C>(a, ny, s, ev);
+    case RK_C2R: return launch_rk_n<N, RK_C2R>(a, ny, s, ev);
   }
   return hipErrorInvalidValue;
 }
@@ -1266,31 +1275,35 @@ int ck_blocks_n() { return (N / 2) / CKShape<N>::RPW; }
 #define GSQG_FOR_EACH_N(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) X(8192)
 
 #if defined(GSQG_INSTANTIATE_N)
-template hipError_t ck_by_job<GSQG_INSTANTIATE_N>(int, const CKArgs&, int, hipStream_t);
-template hipError_t rk_by_job<GSQG_INSTANTIATE_N>(int, const RKArgs&, int, hipStream_t);
+template hipError_t ck_by_job<GSQG_INSTANTIATE_N>(int, const CKArgs&, int, hipStream_t,
+                                                  const LaunchEv*);
+template hipError_t rk_by_job<GSQG_INSTANTIATE_N>(int, const RKArgs&, int, hipStream_t,
+                                                  const LaunchEv*);
 #endif
 
 #if defined(GSQG_INSTANTIATE_COMMON)
 #define GSQG_EXTERN(N_)                                                              \
-  extern template hipError_t ck_by_job<N_>(int, const CKArgs&, int, hipStream_t); \
-  extern template hipError_t rk_by_job<N_>(int, const RKArgs&, int, hipStream_t);
+  extern template hipError_t ck_by_job<N_>(int, const CKArgs&, int, hipStream_t, \
+                                           const LaunchEv*);                     \
+  extern template hipError_t rk_by_job<N_>(int, const RKArgs&, int, hipStream_t, \
+                                           const LaunchEv*);
 GSQG_FOR_EACH_N(GSQG_EXTERN)
 #undef GSQG_EXTERN
 
 #define GSQG_CASE(N_, CALL) \
   case N_: return CALL;
 
-hipError_t launch_ck(int N, int job, const CKArgs& a, int ny, hipStream_t s) {
+hipError_t launch_ck(int N, int job, const CKArgs& a, int ny, hipStream_t s, const LaunchEv* ev) {
   switch (N) {
-#define X(N_) GSQG_CASE(N_, ck_by_job<N_>(job, a, ny, s))
+#define X(N_) GSQG_CASE(N_, ck_by_job<N_>(job, a, ny, s, ev))
     GSQG_FOR_EACH_N(X)
 #undef X
   }
   return hipErrorInvalidValue;
 }
-hipError_t launch_rk(int N, int job, const RKArgs& a, int ny, hipStream_t s) {
+hipError_t launch_rk(int N, int job, const RKArgs& a, int ny, hipStream_t s, const LaunchEv* ev) {
   switch (N) {
-#define X(N_) GSQG_CASE(N_, rk_by_job<N_>(job, a, ny, s))
+#define X(N_) GSQG_CASE(N_, rk_by_job<N_>(job, a, ny, s, ev))
     GSQG_FOR_EACH_N(X)
 #undef X
   }
