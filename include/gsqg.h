@@ -162,6 +162,29 @@ int gsqg_nonfinite(gsqg_plan* plan, int* flag, int clear, void* stream);
  * (synchronises stream); rate4/scale4 are 4-element arrays. */
 int gsqg_forcing_stats(gsqg_plan* plan, double* rate4, double* scale4, void* stream);
 
+/* Spectral diagnostics of a full (n, n) complex128 spectrum X, the sums
+ * behind compute_energy_spectrum (utils/diagnostics.py:21-84),
+ * compute_enstrophy (:119-141), compute_total_energy (:170-190) and
+ * compute_palinstrophy (:193-216).  With |k|^2 = kx^2 + ky^2 (each square
+ * rounded) and X[0,0] excluded from every density:
+ *   bin_sums[b]   = sum of 0.5 |k|^(alpha-2) |X|^2 over edges[b] <= |k| < edges[b+1],
+ *                   b < n_bins (edges: n_bins + 1 host doubles, edges[0] = 0);
+ *   shell_sums[s] = sum of 0.5 |k|^(alpha-2) |X|^2 / n^4 over rint(|k|/dk) == s,
+ *                   s <= n/2 (n/2 + 1 doubles);
+ *   sums[0] = sum 0.5 (k^2)^((alpha-2)/2) |X|^2, sums[1] = sum (|k|^alpha)^2 |X|^2
+ *             (k = 0 contributes 0), sums[2] = sum 0.5 (k^2)^(1+alpha) |X|^2.
+ * Output pointers are host memory; synchronises `stream`. */
+int gsqg_spectral_diagnostics(gsqg_plan* plan, const void* spectrum_dev, double alpha,
+                              const double* edges, int n_bins, double dk, double* bin_sums,
+                              double* shell_sums, double* sums, void* stream);
+
+/* get_diagnostics (core/solver.py:216-245) of a full spectrum X with the
+ * plan's alpha: out[0] = 0.5 mean(u^2 + v^2), out[1] = 0.5 mean(theta^2),
+ * out[2] = max |theta|, theta = ifft2(X).real, (u, v) its velocity.  out is
+ * 3 host doubles; synchronises `stream`. */
+int gsqg_physical_diagnostics(gsqg_plan* plan, const void* spectrum_dev, double* out,
+                              void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -513,6 +513,21 @@ def enstrophy(theta_hat, grid: Grid, alpha: float):
     return float(0.5 * np.mean(np.abs(q) ** 2) * grid.N**2)
 
 
+def palinstrophy(theta_hat, grid: Grid, alpha: float):
+    """diagnostics.py:193-216: sum 0.5 (k2)^(1+alpha) |theta_hat|^2 / N^2, [0,0] = 0."""
+    k2s = np.where(grid.k2 > 0, grid.k2, 1.0)
+    dens = 0.5 * k2s ** (1 + alpha) * np.abs(theta_hat) ** 2
+    dens[0, 0] = 0.0
+    return float(np.sum(dens) / grid.N**2)
+
+
+def energy_flux(theta_hat, forcing_hat, grid: Grid):
+    """diagnostics.py:144-167: Re sum conj(theta_hat) F_hat / N^2."""
+    if forcing_hat is None:
+        return 0.0
+    return float(np.real(np.sum(np.conj(theta_hat) * forcing_hat)) / grid.N**2)
+
+
 def shell_spectrum(theta_hat, grid: Grid, alpha: float):
     """Integer-shell spectrum E(k) = sum over round(|k|/dk) == k of
     0.5*|k|^(alpha-2)*|theta_hat|^2 / N^4 (the closure metric)."""

@@ -12,6 +12,8 @@ from .operators import (compute_streamfunction, compute_velocity_from_theta,
                         fractional_laplacian, gradient, hyperviscosity, jacobian, laplacian,
                         perpendicular_gradient)
 from .solver import gSQGSolver, gSQGSolverOptimized
+from .diagnostics import (compute_energy_flux, compute_energy_spectrum, compute_enstrophy,
+                          compute_palinstrophy, compute_shell_spectrum, compute_total_energy)
 from ._lib import GsqgLibraryError
 
 __all__ = [
@@ -19,6 +21,7 @@ __all__ = [
     "large_scale_damping", "gradient", "laplacian", "fractional_laplacian",
     "perpendicular_gradient", "jacobian", "compute_streamfunction",
     "compute_velocity_from_theta", "hyperviscosity", "gSQGSolver", "gSQGSolverOptimized",
-    "GsqgLibraryError",
+    "compute_energy_spectrum", "compute_total_energy", "compute_enstrophy",
+    "compute_palinstrophy", "compute_shell_spectrum", "compute_energy_flux", "GsqgLibraryError",
 ]
 __version__ = "0.1.0"

@@ -85,6 +85,12 @@ _SIGS = {
     "gsqg_nonfinite": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int), ctypes.c_int, _VP]),
     "gsqg_forcing_stats": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_double), _VP]),
+    "gsqg_spectral_diagnostics": (ctypes.c_int, [_VP, _VP, ctypes.c_double,
+                                                 ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                                 ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                                                 ctypes.POINTER(ctypes.c_double),
+                                                 ctypes.POINTER(ctypes.c_double), _VP]),
+    "gsqg_physical_diagnostics": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(ctypes.c_double), _VP]),
 }
 
 # Every symbol include/gsqg.h declares (checked by the CPU test suite).

@@ -24,6 +24,14 @@ hipError_t launch_a_gather(const double2* X, const int2* modes, double2* aval, i
                            double* norm, hipStream_t s);
 hipError_t launch_a_scatter(double2* X, const int2* modes, const double2* aval, int n, int N,
                             hipStream_t s);
+int diag_blocks();
+hipError_t launch_spec_diag(const double2* X, const double* k1d, const double* edges, int N, int nb,
+                            int nshell, double dk, double alpha, double* part, int stride,
+                            hipStream_t s);
+hipError_t launch_phys_diag(const double* th, const double* u, const double* v, long long n,
+                            double* part, hipStream_t s);
+hipError_t launch_diag_reduce(const double* part, int nblk, int stride, int nsum, double* out,
+                              hipStream_t s);
 hipError_t launch_a_tab(const int2* modes, const double* k1d, const uint8_t* flags, ATab* out, int n,
                         int N, hipStream_t s);
 hipError_t launch_a_rhs(const Phys& ph, const double* k1d, const uint8_t* flags, const Ctrl* ctrl,
@@ -53,6 +61,7 @@ static int fail(int code, const std::string& msg) {
   } while (0)
 
 static const int kMaxScalars = 8;
+static const int kMaxBins = 4096;  // reference energy-spectrum bins per call
 
 struct gsqg_plan {
   int N = 0;
@@ -82,6 +91,10 @@ struct gsqg_plan {
   int n_ring = 0;
   double* norms = nullptr;
   int n_norms = 0;
+  // diagnostics scratch (allocated on first use): partials, reduced slots,
+  // bin edges, a physical field
+  double *d_part = nullptr, *d_out = nullptr, *d_edges = nullptr, *d_phys = nullptr;
+  int d_stride = 0;
   size_t bytes = 0;
   // live kernel timing (gsqg_profile_advance): events around every launch
   bool prof = false;
@@ -113,7 +126,7 @@ static void free_plan(gsqg_plan* p) {
   void* ptrs[] = {p->th,   p->th2,   p->acc,  p->ys,   p->J,     p->F[0], p->F[1], p->F[2],
                   p->F[3], p->s_th,  p->s_acc, p->s_ys, p->s_J,  p->s_F0, p->s_F1, p->uphys,
                   p->vphys, p->k1d,  p->flags, p->tw,   p->ctrl, p->part, p->ring, p->a_tab, p->a_val,
-                  p->a_tmp, p->norms};
+                  p->a_tmp, p->norms, p->d_part, p->d_out, p->d_edges, p->d_phys};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (hipEvent_t e : p->ev_pool) (void)hipEventDestroy(e);
@@ -739,5 +752,73 @@ extern "C" int gsqg_forcing_stats(gsqg_plan* p, double* rate4, double* scale4, v
     if (rate4) rate4[i] = c.rate[i + 1];
     if (scale4) scale4[i] = c.scale[i + 1];
   }
+  return GSQG_OK;
+}
+
+// ------------------------------------------------------------ diagnostics
+static int diag_scratch(gsqg_plan* p, int stride) {
+  if (stride > p->d_stride) {
+    if (p->d_part) (void)hipFree(p->d_part);
+    if (p->d_out) (void)hipFree(p->d_out);
+    p->d_part = p->d_out = nullptr;
+    p->d_stride = 0;
+    HIPCHK(dalloc(p, &p->d_part, (size_t)diag_blocks() * stride));
+    HIPCHK(dalloc(p, &p->d_out, (size_t)stride));
+    p->d_stride = stride;
+  }
+  if (!p->d_edges) HIPCHK(dalloc(p, &p->d_edges, (size_t)kMaxBins + 1));
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_spectral_diagnostics(gsqg_plan* p, const void* X, double alpha,
+                                         const double* edges, int n_bins, double dk,
+                                         double* bin_sums, double* shell_sums, double* sums,
+                                         void* stream) {
+  if (!p || !X || !edges || !bin_sums || !shell_sums || !sums)
+    return fail(GSQG_ERR_ARG, "null argument");
+  if (n_bins < 1 || n_bins > kMaxBins)
+    return fail(GSQG_ERR_ARG, "n_bins must be in [1, " + std::to_string(kMaxBins) + "], got " +
+                                  std::to_string(n_bins));
+  if (!(edges[0] == 0.0) || !(edges[1] > 0.0) || !(dk > 0.0))
+    return fail(GSQG_ERR_ARG, "edges must start at 0 and increase; dk must be positive");
+  hipStream_t st = (hipStream_t)stream;
+  const int N = p->N;
+  const int nshell = N / 2 + 1;
+  const int stride = n_bins + nshell + 3;
+  int rc;
+  if ((rc = diag_scratch(p, stride))) return rc;
+  HIPCHK(hipMemcpyAsync(p->d_edges, edges, (size_t)(n_bins + 1) * sizeof(double),
+                        hipMemcpyHostToDevice, st));
+  HIPCHK(launch_spec_diag((const double2*)X, p->k1d, p->d_edges, N, n_bins, nshell, dk, alpha,
+                          p->d_part, stride, st));
+  HIPCHK(launch_diag_reduce(p->d_part, diag_blocks(), stride, stride, p->d_out, st));
+  std::vector<double> h(stride);
+  HIPCHK(hipMemcpyAsync(h.data(), p->d_out, (size_t)stride * sizeof(double), hipMemcpyDeviceToHost,
+                        st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int b = 0; b < n_bins; ++b) bin_sums[b] = h[b];
+  for (int s = 0; s < nshell; ++s) shell_sums[s] = h[n_bins + s];
+  for (int s = 0; s < 3; ++s) sums[s] = h[n_bins + nshell + s];
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_physical_diagnostics(gsqg_plan* p, const void* X, double* out, void* stream) {
+  if (!p || !X || !out) return fail(GSQG_ERR_ARG, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const int N = p->N;
+  int rc;
+  if ((rc = diag_scratch(p, 3))) return rc;
+  if (!p->d_phys) HIPCHK(dalloc(p, &p->d_phys, (size_t)N * N));
+  if ((rc = gsqg_ifft2(p, X, p->d_phys, stream))) return rc;
+  if ((rc = gsqg_velocity(p, X, p->uphys, p->vphys, stream))) return rc;
+  HIPCHK(launch_phys_diag(p->d_phys, p->uphys, p->vphys, (long long)N * N, p->d_part, st));
+  HIPCHK(launch_diag_reduce(p->d_part, diag_blocks(), 3, 2, p->d_out, st));
+  double h[3];
+  HIPCHK(hipMemcpyAsync(h, p->d_out, sizeof(h), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  const double inv = 1.0 / ((double)N * (double)N);
+  out[0] = 0.5 * (h[1] * inv);
+  out[1] = 0.5 * (h[0] * inv);
+  out[2] = h[2];
   return GSQG_OK;
 }

@@ -1125,6 +1125,130 @@ __global__ void a_scatter_kernel(cd* X, const int2* modes, const cd* aval, int n
   }
 }
 
+// ------------------------------------------------------------ diagnostics
+// Spectral diagnostics of a full (N, N) spectrum X (axis 0 = kx index i,
+// axis 1 = ky index j), utils/diagnostics.py:21-84, 119-141, 170-216.  Per
+// block: the reference's binned energy spectrum (density 0.5 |k|^(alpha-2)
+// |X|^2, X[0,0] excluded, bin b when edges[b] <= |k| < edges[b+1]), the
+// integer-shell spectrum (the same density / N^4 at shell rint(|k|/dk)), and
+// the sums of the total-energy, enstrophy and palinstrophy densities.  |k|^2
+// is formed as kx*kx + ky*ky with both products rounded (no contraction), as
+// the reference's KX**2 + KY**2, so a mode sits in the reference's bin.
+// Output per block at part[blk * stride]: nb bins, nshell shells, 3 sums.
+__global__ void __launch_bounds__(256) spec_diag_kernel(const cd* __restrict__ X,
+                                                        const double* __restrict__ k1d,
+                                                        const double* __restrict__ edges, int N,
+                                                        int nb, int nshell, double dk,
+                                                        double alpha, double* part, int stride) {
+  extern __shared__ double sh[];  // nb + nshell bins, then 3 * 256 sums
+  double* ssum = sh + nb + nshell;
+  const int tid = threadIdx.x;
+  for (int b = tid; b < nb + nshell; b += 256) sh[b] = 0.0;
+  __syncthreads();
+  const double n4 = (double)N * (double)N * (double)N * (double)N;
+  const double step = edges[1];
+  const double kmax = edges[nb];
+  double sE = 0.0, sQ = 0.0, sP = 0.0;
+  const long long total = (long long)N * N;
+  for (long long e = (long long)blockIdx.x * 256 + tid; e < total;
+       e += (long long)gridDim.x * 256) {
+    const int i = (int)(e / N), j = (int)(e - (long long)i * N);
+    const cd x = X[e];
+    const double kx = k1d[i], ky = k1d[j];
+    const double k2 = __dadd_rn(__dmul_rn(kx, kx), __dmul_rn(ky, ky));
+    const double a2 = __dadd_rn(__dmul_rn(x.x, x.x), __dmul_rn(x.y, x.y));
+    const bool dc = (i == 0 && j == 0);
+    const double kmag = sqrt(k2);
+    const double ksafe = kmag > 0.0 ? kmag : 1.0;
+    const double ds = dc ? 0.0 : 0.5 * pow(ksafe, alpha - 2.0) * a2;
+    if (kmag < kmax) {
+      int g = (int)(kmag / step);
+      g = g < 0 ? 0 : (g > nb - 1 ? nb - 1 : g);
+      while (g > 0 && kmag < edges[g]) --g;
+      while (g < nb - 1 && kmag >= edges[g + 1]) ++g;
+      if (kmag >= edges[g] && kmag < edges[g + 1]) atomicAdd(&sh[g], ds);
+    }
+    const double sr = rint(kmag / dk);
+    if (sr < (double)nshell) atomicAdd(&sh[nb + (int)sr], ds / n4);
+    const double k2s = k2 > 0.0 ? k2 : 1.0;
+    if (!dc) {
+      sE += 0.5 * pow(k2s, 0.5 * (alpha - 2.0)) * a2;
+      sP += 0.5 * pow(k2s, 1.0 + alpha) * a2;
+    }
+    if (k2 > 0.0) {
+      const double f = pow(ksafe, alpha);
+      sQ += (f * f) * a2;
+    }
+  }
+  ssum[tid] = sE;
+  ssum[256 + tid] = sQ;
+  ssum[512 + tid] = sP;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (tid < h) {
+      ssum[tid] += ssum[tid + h];
+      ssum[256 + tid] += ssum[256 + tid + h];
+      ssum[512 + tid] += ssum[512 + tid + h];
+    }
+    __syncthreads();
+  }
+  double* out = part + (size_t)blockIdx.x * stride;
+  for (int b = tid; b < nb + nshell; b += 256) out[b] = sh[b];
+  if (tid == 0) {
+    out[nb + nshell] = ssum[0];
+    out[nb + nshell + 1] = ssum[256];
+    out[nb + nshell + 2] = ssum[512];
+  }
+}
+
+// Physical diagnostics (core/solver.py:216-245) of theta, u, v (n doubles
+// each): per block sum theta^2, sum (u^2 + v^2), max |theta|.
+__global__ void __launch_bounds__(256) phys_diag_kernel(const double* __restrict__ th,
+                                                        const double* __restrict__ u,
+                                                        const double* __restrict__ v,
+                                                        long long n, double* part) {
+  __shared__ double s0[256], s1[256], s2[256];
+  const int tid = threadIdx.x;
+  double a = 0.0, b = 0.0, m = 0.0;
+  for (long long e = (long long)blockIdx.x * 256 + tid; e < n; e += (long long)gridDim.x * 256) {
+    const double t = th[e];
+    a += t * t;
+    b += u[e] * u[e] + v[e] * v[e];
+    m = fmax(m, fabs(t));
+  }
+  s0[tid] = a;
+  s1[tid] = b;
+  s2[tid] = m;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (tid < h) {
+      s0[tid] += s0[tid + h];
+      s1[tid] += s1[tid + h];
+      s2[tid] = fmax(s2[tid], s2[tid + h]);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    part[(size_t)blockIdx.x * 3] = s0[0];
+    part[(size_t)blockIdx.x * 3 + 1] = s1[0];
+    part[(size_t)blockIdx.x * 3 + 2] = s2[0];
+  }
+}
+
+// Sum (or max, for slots >= nsum) the per-block partials slot by slot, in
+// block order (deterministic).
+__global__ void __launch_bounds__(256) diag_reduce_kernel(const double* part, int nblk, int stride,
+                                                          int nsum, double* out) {
+  for (int s = blockIdx.x * 256 + threadIdx.x; s < stride; s += gridDim.x * 256) {
+    double acc = 0.0;
+    for (int b = 0; b < nblk; ++b) {
+      const double x = part[(size_t)b * stride + s];
+      acc = s < nsum ? acc + x : fmax(acc, x);
+    }
+    out[s] = acc;
+  }
+}
+
 // Side-state table: wavenumbers and flags of every ring mode, gathered once.
 __global__ void a_tab_kernel(const int2* modes, const double* k1d, const uint8_t* flags, ATab* out,
                              int n, int N) {
@@ -1347,6 +1471,27 @@ hipError_t launch_a_scatter(cd* X, const int2* modes, const cd* aval, int n, int
   int blocks = (n + 255) / 256;
   if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(a_scatter_kernel, dim3(blocks), dim3(256), 0, s, X, modes, aval, n, N);
+  return hipGetLastError();
+}
+int diag_blocks() { return 512; }
+hipError_t launch_spec_diag(const cd* X, const double* k1d, const double* edges, int N, int nb,
+                            int nshell, double dk, double alpha, double* part, int stride,
+                            hipStream_t s) {
+  const size_t shm = (size_t)(nb + nshell + 3 * 256) * sizeof(double);
+  hipLaunchKernelGGL(spec_diag_kernel, dim3(diag_blocks()), dim3(256), shm, s, X, k1d, edges, N,
+                     nb, nshell, dk, alpha, part, stride);
+  return hipGetLastError();
+}
+hipError_t launch_phys_diag(const double* th, const double* u, const double* v, long long n,
+                            double* part, hipStream_t s) {
+  hipLaunchKernelGGL(phys_diag_kernel, dim3(diag_blocks()), dim3(256), 0, s, th, u, v, n, part);
+  return hipGetLastError();
+}
+hipError_t launch_diag_reduce(const double* part, int nblk, int stride, int nsum, double* out,
+                              hipStream_t s) {
+  const int blocks = (stride + 255) / 256;
+  hipLaunchKernelGGL(diag_reduce_kernel, dim3(blocks), dim3(256), 0, s, part, nblk, stride, nsum,
+                     out);
   return hipGetLastError();
 }
 hipError_t launch_a_tab(const int2* modes, const double* k1d, const uint8_t* flags, ATab* out, int n,

@@ -53,9 +53,8 @@ def compute_streamfunction(theta_hat, grid: Grid, alpha: float):
     return _frac_op(grid, -alpha) * theta_hat
 
 
-def compute_velocity_from_theta(theta_hat, grid: Grid, alpha: float):
-    """core/operators.py:158-181, fused: |k|^-alpha and the perpendicular
-    gradient are applied in the column kernel, the C2R in the row kernel."""
+def velocity_plan(grid: Grid, alpha: float):
+    """Shared plan whose multipliers carry |k|^-alpha (velocity, diagnostics)."""
     from .plan import Plan, _PLAN_CACHE_VEL
 
     key = (grid.N, grid.L, str(grid.device), float(alpha))
@@ -63,6 +62,13 @@ def compute_velocity_from_theta(theta_hat, grid: Grid, alpha: float):
     if plan is None:
         plan = Plan(grid, alpha, 0.0, 8)
         _PLAN_CACHE_VEL[key] = plan
+    return plan
+
+
+def compute_velocity_from_theta(theta_hat, grid: Grid, alpha: float):
+    """core/operators.py:158-181, fused: |k|^-alpha and the perpendicular
+    gradient are applied in the column kernel, the C2R in the row kernel."""
+    plan = velocity_plan(grid, alpha)
     th = theta_hat.to(device=grid.device, dtype=torch.complex128).contiguous()
     u = torch.empty((grid.N, grid.N), dtype=torch.float64, device=grid.device)
     v = torch.empty_like(u)

@@ -188,6 +188,26 @@ class Plan:
                                               ctypes.c_uint64(key & 0xFFFFFFFFFFFFFFFF),
                                               _ptr(out), _stream()), "gsqg_ring_forcing")
 
+    def spectral_diagnostics(self, X: torch.Tensor, alpha: float, edges: np.ndarray, dk: float):
+        """(bin_sums, shell_sums, sums) of a full spectrum (gsqg_spectral_diagnostics)."""
+        edges = np.ascontiguousarray(edges, dtype=np.float64)
+        nb = edges.shape[0] - 1
+        bins = np.zeros(nb)
+        shells = np.zeros(self.grid.N // 2 + 1)
+        sums = np.zeros(3)
+        dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        _lib.check(self.lib.gsqg_spectral_diagnostics(self.handle, _ptr(X), float(alpha), dp(edges),
+                                                      nb, float(dk), dp(bins), dp(shells), dp(sums),
+                                                      _stream()), "gsqg_spectral_diagnostics")
+        return bins, shells, sums
+
+    def physical_diagnostics(self, X: torch.Tensor):
+        """(kinetic_energy, enstrophy, max_theta) of a full spectrum."""
+        out = (ctypes.c_double * 3)()
+        _lib.check(self.lib.gsqg_physical_diagnostics(self.handle, _ptr(X), out, _stream()),
+                   "gsqg_physical_diagnostics")
+        return out[0], out[1], out[2]
+
     KERNEL_CLASSES = ("theta_column", "theta_row", "scalar_column", "scalar_row", "other")
 
     def profile_advance(self, dt: float, n_steps: int, key_mode: int, key: int, step0: int = 0):

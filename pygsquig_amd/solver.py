@@ -234,14 +234,14 @@ class gSQGSolver:
         return {"theta_hat": new, "time": t, "step": state["step"] + n_steps}
 
     def get_diagnostics(self, state: State) -> dict:
-        """core/solver.py:216-245."""
-        th = _to_device_c128(state["theta_hat"], self.grid)
-        theta = ifft2(th, self.grid)
-        u, v = self.compute_velocity(th)
-        ke = 0.5 * torch.mean(u**2 + v**2)
-        ens = 0.5 * torch.mean(theta**2)
-        mx = torch.max(torch.abs(theta))
-        return {"kinetic_energy": float(ke), "enstrophy": float(ens), "max_theta": float(mx),
+        """core/solver.py:216-245: kinetic energy 0.5 <u^2 + v^2>, enstrophy
+        0.5 <theta^2> and max |theta| in physical space, reduced on the GPU
+        (gsqg_physical_diagnostics)."""
+        from .operators import velocity_plan
+
+        th = _to_device_c128(state["theta_hat"], self.grid).contiguous()
+        ke, ens, mx = velocity_plan(self.grid, self.alpha).physical_diagnostics(th)
+        return {"kinetic_energy": ke, "enstrophy": ens, "max_theta": mx,
                 "time": float(state["time"]), "step": int(state["step"])}
 
     def check_finite(self, forcing=None, damping=None) -> bool:

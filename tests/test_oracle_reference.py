@@ -311,6 +311,18 @@ def test_diagnostics_single_mode():
     shells = R.shell_spectrum(st["theta_hat"], g, 1.0)
     np.testing.assert_allclose(shells[3], 0.5 / 3 * 0.5, rtol=1e-13)
     assert shells.sum() == pytest.approx(shells[3])
+    # two modes (+-3, 0) of |theta_hat| = N^2 / 2, alpha = 1:
+    # enstrophy 0.5 * 2 * 3^2 * N^4/4; palinstrophy 2 * 0.5 * (3^2)^2 * N^4/4 / N^2
+    np.testing.assert_allclose(R.enstrophy(st["theta_hat"], g, 1.0), 9 * N**4 / 4, rtol=1e-13)
+    np.testing.assert_allclose(R.palinstrophy(st["theta_hat"], g, 1.0), 81 * N**2 / 4, rtol=1e-13)
+    # binned spectrum: N//4 = 8 bins of width 2 over [0, 16]; |k| = 3 in bin 1
+    kb, ek = R.energy_spectrum(st["theta_hat"], g, 1.0)
+    np.testing.assert_allclose(kb, np.arange(1.0, 16.0, 2.0), rtol=1e-15)
+    np.testing.assert_allclose(ek[1], 2 * 0.5 / 3 * N**4 / 4 / 2, rtol=1e-13)
+    assert np.max(np.abs(np.delete(ek, 1))) < 1e-25 * ek[1]  # FFT roundoff only
+    # energy flux <theta F> of F = theta_hat: sum |theta_hat|^2 / N^2 = 2 (N^2/2)^2 / N^2
+    np.testing.assert_allclose(R.energy_flux(st["theta_hat"], st["theta_hat"], g), N**2 / 2,
+                               rtol=1e-13)
 
 
 # --------------------------------------------- planted mistakes are caught
