@@ -74,6 +74,12 @@ typedef struct gsqg_plan_desc {
   const uint8_t* mode_flags;
   const int32_t* ring_modes;
   int n_ring;
+  /* Ensemble: n_members >= 1 independent realizations of this grid size,
+   * advanced together (no forcing, no scalars when n_members > 1);
+   * member_alpha[n_members] gives each member's gSQG exponent (NULL: alpha
+   * for all).  Replaces looping gSQGSolver over realizations. */
+  int n_members;
+  const double* member_alpha;
 } gsqg_plan_desc;
 
 /* Library version (major*10000 + minor*100 + patch). */
@@ -102,6 +108,11 @@ int gsqg_load_theta(gsqg_plan* plan, const void* theta_hat_dev, double* offring_
 /* Write the plan's theta_hat as a full (n, n) complex128 spectrum.
  * Output side of gSQGSolver.step (core/solver.py:214). */
 int gsqg_store_theta(gsqg_plan* plan, void* theta_hat_dev, void* stream);
+
+/* Same for ensemble member `idx` of a plan with n_members > 1 (member 0 is
+ * what gsqg_load_theta / gsqg_store_theta address). */
+int gsqg_load_member(gsqg_plan* plan, int idx, const void* theta_hat_dev, void* stream);
+int gsqg_store_member(gsqg_plan* plan, int idx, void* theta_hat_dev, void* stream);
 
 /* Same for passive scalar `idx` (scalars/state.py:59-118 MultiScalarState.scalars). */
 int gsqg_load_scalar(gsqg_plan* plan, int idx, const void* scalar_hat_dev, double* offring_sq,

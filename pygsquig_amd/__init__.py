@@ -14,6 +14,7 @@ from .operators import (compute_streamfunction, compute_velocity_from_theta,
 from .solver import gSQGSolver, gSQGSolverOptimized
 from .diagnostics import (compute_energy_flux, compute_energy_spectrum, compute_enstrophy,
                           compute_palinstrophy, compute_shell_spectrum, compute_total_energy)
+from .ensemble import EnsembleSolver
 from ._lib import GsqgLibraryError
 
 __all__ = [
@@ -22,6 +23,7 @@ __all__ = [
     "perpendicular_gradient", "jacobian", "compute_streamfunction",
     "compute_velocity_from_theta", "hyperviscosity", "gSQGSolver", "gSQGSolverOptimized",
     "compute_energy_spectrum", "compute_total_energy", "compute_enstrophy",
-    "compute_palinstrophy", "compute_shell_spectrum", "compute_energy_flux", "GsqgLibraryError",
+    "compute_palinstrophy", "compute_shell_spectrum", "compute_energy_flux", "EnsembleSolver",
+    "GsqgLibraryError",
 ]
 __version__ = "0.1.0"

@@ -55,6 +55,8 @@ class GsqgPlanDesc(ctypes.Structure):
         ("mode_flags", ctypes.POINTER(ctypes.c_uint8)),
         ("ring_modes", ctypes.POINTER(ctypes.c_int32)),
         ("n_ring", ctypes.c_int),
+        ("n_members", ctypes.c_int),
+        ("member_alpha", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -68,6 +70,8 @@ _SIGS = {
     "gsqg_plan_device_bytes": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_size_t)]),
     "gsqg_load_theta": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(ctypes.c_double), _VP]),
     "gsqg_store_theta": (ctypes.c_int, [_VP, _VP, _VP]),
+    "gsqg_load_member": (ctypes.c_int, [_VP, ctypes.c_int, _VP, _VP]),
+    "gsqg_store_member": (ctypes.c_int, [_VP, ctypes.c_int, _VP, _VP]),
     "gsqg_load_scalar": (ctypes.c_int, [_VP, ctypes.c_int, _VP, ctypes.POINTER(ctypes.c_double), _VP]),
     "gsqg_store_scalar": (ctypes.c_int, [_VP, ctypes.c_int, _VP, _VP]),
     "gsqg_advance": (ctypes.c_int, [_VP, ctypes.c_double, ctypes.c_int, ctypes.c_int,

@@ -70,6 +70,8 @@ struct gsqg_plan {
   Phys ph{};
   int n_scalars = 0;
   double kappa[kMaxScalars] = {0};
+  int n_mem = 1;              // ensemble members (theta fields, stride hsz)
+  double* malpha = nullptr;   // per-member alpha (device), n_mem > 1 only
   // theta
   double2 *th = nullptr, *th2 = nullptr, *acc = nullptr, *ys = nullptr, *J = nullptr;
   double2* F[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -126,7 +128,7 @@ static void free_plan(gsqg_plan* p) {
   void* ptrs[] = {p->th,   p->th2,   p->acc,  p->ys,   p->J,     p->F[0], p->F[1], p->F[2],
                   p->F[3], p->s_th,  p->s_acc, p->s_ys, p->s_J,  p->s_F0, p->s_F1, p->uphys,
                   p->vphys, p->k1d,  p->flags, p->tw,   p->ctrl, p->part, p->ring, p->a_tab, p->a_val,
-                  p->a_tmp, p->norms, p->d_part, p->d_out, p->d_edges, p->d_phys};
+                  p->a_tmp, p->norms, p->d_part, p->d_out, p->d_edges, p->d_phys, p->malpha};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (hipEvent_t e : p->ev_pool) (void)hipEventDestroy(e);
@@ -157,6 +159,15 @@ extern "C" int gsqg_plan_create(const gsqg_plan_desc* d, gsqg_plan** out) {
   if (d->p != 2 && d->p != 4 && d->p != 8) return fail(GSQG_ERR_ARG, "p must be 2, 4, or 8");
   if (d->damping && d->damp_p != 2 && d->damp_p != 4 && d->damp_p != 8)
     return fail(GSQG_ERR_ARG, "damping p must be 2, 4, or 8");
+  const int nmem = d->n_members > 0 ? d->n_members : 1;
+  if (nmem > 1 && (d->forcing || d->n_scalars > 0))
+    return fail(GSQG_ERR_UNSUPPORTED, "ensembles (n_members > 1) carry no forcing and no scalars");
+  if (nmem > 1 && d->member_alpha) {
+    for (int m = 0; m < nmem; ++m)
+      if (!(d->member_alpha[m] >= -2.0 && d->member_alpha[m] <= 2.0))
+        return fail(GSQG_ERR_ARG, "member alpha must be in [-2, 2], got " +
+                                      std::to_string(d->member_alpha[m]));
+  }
   if (d->n_scalars < 0 || d->n_scalars > kMaxScalars)
     return fail(GSQG_ERR_UNSUPPORTED, "at most 8 passive scalars");
   if (!d->k1d || !d->mode_flags) return fail(GSQG_ERR_ARG, "k1d and mode_flags are required");
@@ -199,12 +210,15 @@ extern "C" int gsqg_plan_create(const gsqg_plan_desc* d, gsqg_plan** out) {
     }                                                          \
   } while (0)
   const size_t h = p->hsz;
-  ALLOC(p->th, h);
+  p->n_mem = nmem;
+  const size_t hm = h * (size_t)nmem;
+  ALLOC(p->th, hm);
   ALLOC(p->th2, h);
-  ALLOC(p->acc, h);
-  ALLOC(p->ys, h);
-  ALLOC(p->J, h);
-  for (int q = 0; q < 4; ++q) ALLOC(p->F[q], h);
+  ALLOC(p->acc, hm);
+  ALLOC(p->ys, hm);
+  ALLOC(p->J, hm);
+  for (int q = 0; q < 4; ++q) ALLOC(p->F[q], hm);
+  if (nmem > 1) ALLOC(p->malpha, (size_t)nmem);
   if (p->n_scalars > 0) {
     const size_t hs = h * p->n_scalars;
     ALLOC(p->s_th, hs);
@@ -253,7 +267,13 @@ extern "C" int gsqg_plan_create(const gsqg_plan_desc* d, gsqg_plan** out) {
   Ctrl c0{};
   for (int i = 0; i < 8; ++i) c0.scale[i] = 1.0;
   HIPCHK(hipMemcpy(p->ctrl, &c0, sizeof(Ctrl), hipMemcpyHostToDevice));
-  HIPCHK(hipMemset(p->th, 0, h * sizeof(double2)));
+  HIPCHK(hipMemset(p->th, 0, hm * sizeof(double2)));
+  if (nmem > 1) {
+    std::vector<double> al(nmem, d->alpha);
+    if (d->member_alpha)
+      for (int m = 0; m < nmem; ++m) al[m] = d->member_alpha[m];
+    HIPCHK(hipMemcpy(p->malpha, al.data(), nmem * sizeof(double), hipMemcpyHostToDevice));
+  }
   if (p->n_scalars > 0) HIPCHK(hipMemset(p->s_th, 0, h * p->n_scalars * sizeof(double2)));
   HIPCHK(hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking));
   HIPCHK(hipStreamCreateWithFlags(&p->cap2, hipStreamNonBlocking));
@@ -316,6 +336,8 @@ static hipError_t rk(gsqg_plan* p, int job, const RKArgs& a, int ny, hipStream_t
 }
 
 // ----------------------------------------------------------- arg builders
+// Column-kernel arguments for theta.  On the plan's state (th == p->th) an
+// ensemble's members are the y-slices of one launch (ny = n_mem).
 static CKArgs ck_theta(gsqg_plan* p, double2* th, int e, int prologue, int rate_slot,
                        int rate_next, double dt) {
   CKArgs a{};
@@ -340,6 +362,10 @@ static CKArgs ck_theta(gsqg_plan* p, double2* th, int e, int prologue, int rate_
   a.a_val = p->a_val;
   a.n_a = p->n_ring;
   a.field_stride = 0;
+  if (th == p->th && p->n_mem > 1) {
+    a.field_stride = (long long)p->hsz;
+    a.member_alpha = p->malpha;
+  }
   return a;
 }
 
@@ -357,7 +383,7 @@ static RKArgs rk_theta(gsqg_plan* p, int red_slot, int inc_step, bool uv_out) {
   a.red_slot = p->ph.forcing ? red_slot : 0;
   a.inc_step = inc_step;
   a.eps = p->ph.eps;
-  a.field_stride = 0;
+  a.field_stride = (long long)p->hsz;  // ensemble members; 0 .. n_mem - 1 as y-slices
   return a;
 }
 
@@ -436,12 +462,12 @@ static int theta_rest(gsqg_plan* p, double dt, bool derived, bool last_prologue,
                       hipStream_t st) {
   const int N = p->N;
   for (int e = 1; e <= 3; ++e) {
-    HIPCHK(ck(p, CK_THETA, ck_theta(p, p->th, e, 1, e + 1, 0, dt), 1, st));
-    HIPCHK(rk(p, RK_THETA, rk_theta(p, e + 1, 0, false), 1, st));
+    HIPCHK(ck(p, CK_THETA, ck_theta(p, p->th, e, 1, e + 1, 0, dt), p->n_mem, st));
+    HIPCHK(rk(p, RK_THETA, rk_theta(p, e + 1, 0, false), p->n_mem, st));
   }
   HIPCHK(ck(p, CK_THETA,
                    ck_theta(p, p->th, 4, last_prologue ? 1 : 0, last_rate, derived ? 1 : 0, dt),
-                   1, st));
+                   p->n_mem, st));
   return GSQG_OK;
 }
 
@@ -456,7 +482,7 @@ static int theta_rest(gsqg_plan* p, double dt, bool derived, bool last_prologue,
 static int step_body(gsqg_plan* p, double dt, bool derived, bool scalars_pending,
                      bool last_prologue, int last_rate, bool scalar_next_prologue,
                      hipStream_t st, hipStream_t side = nullptr) {
-  HIPCHK(rk(p, RK_THETA, rk_theta(p, 1, derived ? 1 : 0, scalars_pending), 1, st));
+  HIPCHK(rk(p, RK_THETA, rk_theta(p, 1, derived ? 1 : 0, scalars_pending), p->n_mem, st));
   int rc;
   if (scalars_pending && side) {
     HIPCHK(hipEventRecord(p->ev_fork, st));
@@ -485,14 +511,14 @@ extern "C" int gsqg_advance(gsqg_plan* p, double dt, int n_steps, int key_mode, 
   int rc;
   HIPCHK(launch_set_ctrl(p->ctrl, derived ? step0 - 1 : step0, key, key, derived ? 1 : 0, st));
   // Prologue of the first stage.
-  HIPCHK(ck(p, CK_THETA, ck_theta(p, p->th, 0, 1, 1, derived ? 1 : 0, dt), 1, st));
+  HIPCHK(ck(p, CK_THETA, ck_theta(p, p->th, 0, 1, 1, derived ? 1 : 0, dt), p->n_mem, st));
   if (sc) {
     if ((rc = launch_scalar_ck(p, 0, 1, dt, st))) return rc;
   }
   // step 0
   {
     const bool last = (n_steps == 1);
-    HIPCHK(rk(p, RK_THETA, rk_theta(p, 1, derived ? 1 : 0, false), 1, st));
+    HIPCHK(rk(p, RK_THETA, rk_theta(p, 1, derived ? 1 : 0, false), p->n_mem, st));
     if ((rc = theta_rest(p, dt, derived, !last || sc, last ? 0 : 1, st))) return rc;
   }
   // middle steps 1 .. n-2
@@ -579,6 +605,24 @@ extern "C" int gsqg_load_theta(gsqg_plan* p, const void* X, double* offring_sq, 
   return load_half(p, (const double2*)X, p->th, offring_sq, true, (hipStream_t)stream);
 }
 
+extern "C" int gsqg_load_member(gsqg_plan* p, int idx, const void* X, void* stream) {
+  if (!p || !X) return fail(GSQG_ERR_ARG, "null argument");
+  if (idx < 0 || idx >= p->n_mem)
+    return fail(GSQG_ERR_ARG, "member index " + std::to_string(idx) + " out of range [0, " +
+                                  std::to_string(p->n_mem) + ")");
+  return load_half(p, (const double2*)X, p->th + (size_t)idx * p->hsz, nullptr, idx == 0,
+                   (hipStream_t)stream);
+}
+extern "C" int gsqg_store_member(gsqg_plan* p, int idx, void* X, void* stream) {
+  if (!p || !X) return fail(GSQG_ERR_ARG, "null argument");
+  if (idx < 0 || idx >= p->n_mem)
+    return fail(GSQG_ERR_ARG, "member index " + std::to_string(idx) + " out of range [0, " +
+                                  std::to_string(p->n_mem) + ")");
+  HIPCHK(launch_half_to_full(p->th + (size_t)idx * p->hsz, (double2*)X, p->N, (hipStream_t)stream));
+  if (idx == 0 && p->n_ring > 0)
+    HIPCHK(launch_a_scatter((double2*)X, p->ring, p->a_val, p->n_ring, p->N, (hipStream_t)stream));
+  return GSQG_OK;
+}
 extern "C" int gsqg_store_theta(gsqg_plan* p, void* X, void* stream) {
   if (!p || !X) return fail(GSQG_ERR_ARG, "null argument");
   hipStream_t st = (hipStream_t)stream;

@@ -89,6 +89,7 @@ struct CKArgs {
   int rate_next_key;     // prologue rate uses the key of step+1
   double dt;
   Phys ph;
+  const double* member_alpha;  // per-field gSQG exponent (ensembles; null: ph.alpha)
   const ATab* a_tab;     // anti-Hermitian ring side state: mode of each entry
   double2* a_val;
   int n_a;

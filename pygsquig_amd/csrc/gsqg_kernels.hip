@@ -68,9 +68,12 @@ __device__ __forceinline__ double hv_factor(double k2, int p) {
 }
 
 // |k|^-alpha, zero at k = 0 (core/operators.py:149-153).
-__device__ __forceinline__ double inv_frac(double k2, const Phys& ph) {
+__device__ __forceinline__ int alpha_kind_of(double alpha) {
+  return alpha == 1.0 ? 1 : alpha == 0.0 ? 2 : alpha == 2.0 ? 3 : 0;
+}
+__device__ __forceinline__ double inv_frac(double k2, double alpha, int kind) {
   if (!(k2 > 0.0)) return 0.0;
-  switch (ph.alpha_kind) {
+  switch (kind) {
     case 1:
       return 1.0 / sqrt(k2);
     case 2:
@@ -78,7 +81,7 @@ __device__ __forceinline__ double inv_frac(double k2, const Phys& ph) {
     case 3:
       return 1.0 / k2;
     default:
-      return pow(sqrt(k2), -ph.alpha);
+      return pow(sqrt(k2), -alpha);
   }
 }
 
@@ -663,6 +666,9 @@ __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
   constexpr int NF = NFields<JOB>::value;
   constexpr int G = S::FP < NF ? S::FP : NF;
   constexpr int RG = S::REGION;
+  // gSQG exponent of this field: per ensemble member (blockIdx.y) when given
+  const double alpha = a.member_alpha ? a.member_alpha[blockIdx.y] : ph.alpha;
+  const int akind = a.member_alpha ? alpha_kind_of(alpha) : ph.alpha_kind;
 #pragma unroll
   for (int g0 = 0; g0 < NF; g0 += G) {
     cd w[G][8];
@@ -671,7 +677,7 @@ __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
     for (int m = 0; m < 8; ++m) {
       const bool selfc = row0 && t == 0 && (m == 0 || m == 4);  // their fields vanish
       const MInfo mi = minfo<N>(ctx, m, t, j, row0);
-      const double K = needK ? inv_frac(mi.kx * mi.kx + mi.ky * mi.ky, ph) : 0.0;
+      const double K = needK ? inv_frac(mi.kx * mi.kx + mi.ky * mi.ky, alpha, akind) : 0.0;
 #pragma unroll
       for (int g = 0; g < G; ++g)
         w[g][m] = selfc ? cmk(0.0, 0.0)

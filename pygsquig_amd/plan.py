@@ -69,7 +69,7 @@ class Plan:
     def __init__(self, grid: Grid, alpha: float = 1.0, nu_p: float = 0.0, p: int = 8, *,
                  scheme: int = _lib.SCHEME_EXPLICIT, combine: int = _lib.COMBINE_STEP,
                  hv_dc_zero: bool = True, damping=None, forcing=None,
-                 kappas: Sequence[float] = ()):
+                 kappas: Sequence[float] = (), member_alphas: Optional[Sequence[float]] = None):
         self.lib = _lib.load()
         self.grid = grid
         N, L = grid.N, grid.L
@@ -115,6 +115,10 @@ class Plan:
         d.mode_flags = self.flags.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
         d.ring_modes = self.ring_modes.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
         d.n_ring = int(self.ring_modes.shape[0])
+        self.member_alphas = tuple(float(a) for a in member_alphas) if member_alphas else (alpha,)
+        self._malpha = (ctypes.c_double * len(self.member_alphas))(*self.member_alphas)
+        d.n_members = len(self.member_alphas)
+        d.member_alpha = self._malpha
         handle = ctypes.c_void_p()
         with torch.cuda.device(grid.device):
             _lib.check(self.lib.gsqg_plan_create(ctypes.byref(d), ctypes.byref(handle)),
@@ -143,6 +147,15 @@ class Plan:
 
     def store_theta(self, out: torch.Tensor) -> None:
         _lib.check(self.lib.gsqg_store_theta(self.handle, _ptr(out), _stream()), "gsqg_store_theta")
+
+    def load_member(self, idx: int, th: torch.Tensor) -> None:
+        _lib.check(self.lib.gsqg_load_member(self.handle, idx, _ptr(th), _stream()),
+                   "gsqg_load_member")
+        self.state_ref = None
+
+    def store_member(self, idx: int, out: torch.Tensor) -> None:
+        _lib.check(self.lib.gsqg_store_member(self.handle, idx, _ptr(out), _stream()),
+                   "gsqg_store_member")
 
     def load_scalar(self, idx: int, s: torch.Tensor, check: bool = True) -> float:
         off = ctypes.c_double(0.0)
