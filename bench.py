@@ -10,6 +10,16 @@ integrating-factor scheme: the explicit scheme is unstable at this nu (the
 timed state would be non-finite within tens of steps).
 
 Prints one JSON line; see the keys below.
+
+``--config`` selects the other configurations of the reference's benchmark
+set (each its own JSON line, same keys):
+  plain  N=256, alpha=1, nu_p=1e-4, p=8, theta0 = sin(4x)cos(4y), dt=1e-3
+         (integrating-factor scheme: explicit RK4 is unstable at this nu);
+  large  N=8192, alpha=0.5, k^-5/3 theta0, ring kf=100 dk=2 eps=0.1,
+         mu=0.1, nu_p=1e-8*(512/N)^8, p=8, 4 Gaussian scalars, dt=1e-4;
+  batch  96 members, 32 each at N=128, 256, 512, alpha cycling
+         {0, 0.5, 1, 1.5, 2}, seeded k^-5/3 theta0, nu_p=1e-4, p=8, dt=1e-3,
+         against looping the single-realization solver over the members.
 """
 
 from __future__ import annotations
@@ -72,6 +82,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--copy-only", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates PMC byte counters)")
+    ap.add_argument("--config", default="forced", choices=["forced", "plain", "large", "batch"])
     args = ap.parse_args()
     if args.copy_only:
         import torch
@@ -88,6 +99,10 @@ def main():
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU path is "
               "not part of this build", file=sys.stderr)
         sys.exit(2)
+    if args.config == "batch":
+        return bench_batch(args)
+    if args.config in ("plain", "large"):
+        return bench_single(args)
 
     import numpy as np
     import torch
@@ -185,6 +200,128 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def _timed(fn, steps):
+    """Run fn() once between HIP events and wall clocks; seconds (the larger)."""
+    import torch
+
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    out = fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return out, max(time.perf_counter() - t0, e0.elapsed_time(e1) / 1e3)
+
+
+def bench_single(args):
+    """Configurations 'plain' (N=256) and 'large' (N=8192)."""
+    import numpy as np
+    import torch
+
+    import pygsquig_amd as pg
+    from pygsquig_amd import ics
+    from pygsquig_amd.scalars import gSQGSolverWithScalars
+
+    dev = torch.device("cuda:0")
+    if args.config == "plain":
+        N, alpha, dt, n_s = 256, 1.0, 1e-3, 0
+        steps = args.steps if args.steps != 2000 else 1000
+        g = pg.make_grid(N, 2 * np.pi, dev)
+        solver = pg.gSQGSolverOptimized(g, alpha, 1e-4, 8, scheme="if")
+        st = solver.initialize(torch.from_numpy(ics.sin4(N)).to(dev))
+        run = lambda s, n: solver.multistep(s, n, dt)
+        workload = "plain unforced gSQG N=256, theta0 = sin(4x)cos(4y), nu_p=1e-4, p=8, dt=1e-3"
+    else:
+        N, alpha, dt, n_s = 8192, 0.5, 1e-4, 4
+        steps = args.steps if args.steps != 2000 else 20
+        g = pg.make_grid(N, 2 * np.pi, dev)
+        solver = gSQGSolverWithScalars(g, alpha, 0.0, 8, scheme="if",
+                                       passive_scalars={f"s{i}": {"kappa": 0.0} for i in range(n_s)})
+        f = pg.RingForcing(kf=100, dk=2, epsilon=0.1, seed=42)
+        d = pg.CombinedDamping(mu=0.1, kf=100, nu_p=1e-8 * (512 / N) ** 8, p=8)
+        init = {f"s{i}": torch.from_numpy(ics.gaussian_blob(N, c, 2 * np.pi / 16)).to(dev)
+                for i, c in enumerate(ics.SCALAR_CENTRES[:n_s])}
+        st = solver.initialize(torch.from_numpy(ics.kolmogorov_53(N, seed=2, alpha=alpha)).to(dev),
+                               scalar_init=init)
+        run = lambda s, n: solver.multistep(s, n, dt, forcing=f, damping=d)
+        workload = ("large forced gSQG N=8192, alpha=0.5, ring kf=100, 4 passive scalars, "
+                    "dt=1e-4")
+    warm = min(args.warmup, 5) if args.config == "large" else args.warmup
+    st = run(st, warm)
+    st, sec = _timed(lambda: run(st, steps), steps)
+    finite = bool(torch.isfinite(st["theta_hat"]).all())
+    sps = steps / sec
+    B = step_bytes(N, n_s)
+    print(json.dumps({
+        "metric": "N^2*RK4-steps/s + HBM GB/s fp64", "value": N * N * sps,
+        "unit": "grid-point RK4 steps/s", "n_gpus": 1, "steps": steps, "warmup": warm,
+        "ms_per_step": sec / steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": workload, "N": N, "alpha": alpha, "scheme": "if", "scalars": n_s,
+                   "steps_per_s": sps, "finite": finite},
+        "step_roofline": {"bound": "hbm", "achieved": B * sps / 1e9, "peak": PEAK_HBM / 1e9,
+                          "unit": "GB/s", "frac": B * sps / PEAK_HBM, "bytes_per_step": B},
+        "cpu_baseline": None,
+        "note": "cpu_baseline is reported for the default configuration only"}))
+
+
+def bench_batch(args):
+    """96 uneven members in batched plans, against looping the
+    single-realization solver over the same members on the same GPU."""
+    import numpy as np
+    import torch
+
+    import pygsquig_amd as pg
+    from pygsquig_amd import ics
+
+    dev = torch.device("cuda:0")
+    alphas = (0.0, 0.5, 1.0, 1.5, 2.0)
+    members = [(N, alphas[m % 5]) for m, N in enumerate([128] * 32 + [256] * 32 + [512] * 32)]
+    dt, nu = 1e-3, 1e-4
+    steps = args.steps if args.steps != 2000 else 500
+    thetas = [torch.from_numpy(ics.kolmogorov_53(N, seed=100 + m, alpha=a)).to(dev)
+              for m, (N, a) in enumerate(members)]
+    ens = pg.EnsembleSolver(members, nu_p=nu, p=8, scheme="if", device=dev)
+    st = ens.initialize(thetas)
+    st = ens.multistep(st, args.warmup, dt)
+    st, sec = _timed(lambda: ens.multistep(st, steps, dt), steps)
+    finite = all(bool(torch.isfinite(t).all()) for t in st["theta_hat"])
+    pts = sum(N * N for N, _ in members)
+    value = pts * steps / sec
+    # baseline: the single-realization solver looped over the members
+    solvers = {}
+    states = []
+    for (N, a), th in zip(members, thetas):
+        key = (N, a)
+        if key not in solvers:
+            solvers[key] = pg.gSQGSolverOptimized(pg.make_grid(N, 2 * np.pi, dev), a, nu, 8,
+                                                  scheme="if")
+        states.append(solvers[key].initialize(th))
+    states = [solvers[k].multistep(s, args.warmup, dt) for k, s in zip(members, states)]
+    loop_steps = max(steps // 5, 1)
+    _, lsec = _timed(lambda: [solvers[k].multistep(s, loop_steps, dt)
+                              for k, s in zip(members, states)], loop_steps)
+    loop_value = pts * loop_steps / lsec
+    B = sum(step_bytes(N, 0) for N, _ in members)
+    print(json.dumps({
+        "metric": "member-grid-point RK4 steps/s (batched uneven ensemble)", "value": value,
+        "unit": "member-grid-point RK4 steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": sec / steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "96 members: 32 each at N=128, 256, 512; alpha cycling "
+                               "{0, 0.5, 1, 1.5, 2}; nu_p=1e-4, p=8, dt=1e-3, IF scheme",
+                   "members": len(members), "finite": finite},
+        "step_roofline": {"bound": "hbm", "achieved": B * steps / sec / 1e9,
+                          "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                          "frac": B * steps / sec / PEAK_HBM, "bytes_per_step": B},
+        "loop_baseline": {"value": loop_value, "unit": "member-grid-point RK4 steps/s",
+                          "steps": loop_steps, "speedup": value / loop_value},
+        "cpu_baseline": None,
+        "note": "cpu_baseline is reported for the default configuration only"}))
 
 
 if __name__ == "__main__":
