@@ -61,6 +61,7 @@ static int fail(int code, const std::string& msg) {
   } while (0)
 
 static const int kMaxScalars = 8;
+static const int kGraphSteps = 8;  // steps per replayed graph (amortises the launch)
 static const int kMaxBins = 4096;  // reference energy-spectrum bins per call
 
 struct gsqg_plan {
@@ -108,7 +109,8 @@ struct gsqg_plan {
   hipStream_t cap = nullptr;
   hipStream_t cap2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  hipGraphExec_t gexec = nullptr;
+  hipGraphExec_t gexec = nullptr;     // one captured step
+  hipGraphExec_t gexec_k = nullptr;   // kGraphSteps captured steps
   double g_dt = 0.0;
   int g_key_mode = -1;
   uint64_t g_key = 0;
@@ -133,6 +135,7 @@ static void free_plan(gsqg_plan* p) {
     if (q) (void)hipFree(q);
   for (hipEvent_t e : p->ev_pool) (void)hipEventDestroy(e);
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+  if (p->gexec_k) (void)hipGraphExecDestroy(p->gexec_k);
   if (p->cap) (void)hipStreamDestroy(p->cap);
   if (p->cap2) (void)hipStreamDestroy(p->cap2);
   if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
@@ -499,6 +502,24 @@ static int step_body(gsqg_plan* p, double dt, bool derived, bool scalars_pending
   return theta_rest(p, dt, derived, last_prologue, last_rate, st);
 }
 
+// Capture nsteps steady-state step bodies into one executable graph.
+static int capture_steps(gsqg_plan* p, int nsteps, double dt, bool derived, bool sc,
+                         hipGraphExec_t* out) {
+  hipGraph_t g;
+  HIPCHK(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
+  int rc = GSQG_OK;
+  for (int s = 0; s < nsteps && rc == GSQG_OK; ++s)
+    rc = step_body(p, dt, derived, sc, true, 1, true, p->cap, p->cap2);
+  hipError_t ce = hipStreamEndCapture(p->cap, &g);
+  if (rc) return rc;
+  if (ce != hipSuccess) return fail(GSQG_ERR_HIP, std::string("capture: ") + hipGetErrorString(ce));
+  hipError_t ie = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ie != hipSuccess)
+    return fail(GSQG_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+  return GSQG_OK;
+}
+
 extern "C" int gsqg_advance(gsqg_plan* p, double dt, int n_steps, int key_mode, uint64_t key,
                             int64_t step0, int use_graph, void* stream) {
   if (!p) return fail(GSQG_ERR_ARG, "null plan");
@@ -525,24 +546,21 @@ extern "C" int gsqg_advance(gsqg_plan* p, double dt, int n_steps, int key_mode, 
   const int n_mid = n_steps >= 2 ? n_steps - 2 : 0;
   if (n_mid > 0) {
     if (use_graph && n_mid >= 2) {
-      if (!p->gexec || p->g_dt != dt || p->g_key_mode != key_mode || p->g_key != key) {
-        if (p->gexec) {
-          HIPCHK(hipGraphExecDestroy(p->gexec));
-          p->gexec = nullptr;
-        }
-        hipGraph_t g;
-        HIPCHK(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
-        rc = step_body(p, dt, derived, sc, true, 1, true, p->cap, p->cap2);
-        hipError_t ce = hipStreamEndCapture(p->cap, &g);
-        if (rc) return rc;
-        if (ce != hipSuccess) return fail(GSQG_ERR_HIP, std::string("capture: ") + hipGetErrorString(ce));
-        HIPCHK(hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0));
-        HIPCHK(hipGraphDestroy(g));
+      if (p->g_dt != dt || p->g_key_mode != key_mode || p->g_key != key) {
+        if (p->gexec) HIPCHK(hipGraphExecDestroy(p->gexec));
+        if (p->gexec_k) HIPCHK(hipGraphExecDestroy(p->gexec_k));
+        p->gexec = p->gexec_k = nullptr;
         p->g_dt = dt;
         p->g_key_mode = key_mode;
         p->g_key = key;
       }
-      for (int s = 0; s < n_mid; ++s) HIPCHK(hipGraphLaunch(p->gexec, st));
+      const int nk = n_mid / kGraphSteps, n1 = n_mid % kGraphSteps;
+      if (nk > 0 && !p->gexec_k)
+        if ((rc = capture_steps(p, kGraphSteps, dt, derived, sc, &p->gexec_k))) return rc;
+      if (n1 > 0 && !p->gexec)
+        if ((rc = capture_steps(p, 1, dt, derived, sc, &p->gexec))) return rc;
+      for (int s = 0; s < nk; ++s) HIPCHK(hipGraphLaunch(p->gexec_k, st));
+      for (int s = 0; s < n1; ++s) HIPCHK(hipGraphLaunch(p->gexec, st));
     } else {
       for (int s = 0; s < n_mid; ++s) {
         if ((rc = step_body(p, dt, derived, sc, true, 1, true, st))) return rc;
