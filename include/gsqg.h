@@ -127,8 +127,10 @@ int gsqg_store_scalar(gsqg_plan* plan, int idx, void* scalar_hat_dev, void* stre
  * .step (core/solver_with_scalars.py:131-178).
  * key_mode GSQG_KEY_EXPLICIT: `key` drives the forcing phases of every step
  * (the reference passes one key per step); GSQG_KEY_DERIVED: step s uses
- * step_key(key, step0 + s).  use_graph != 0 captures one step in a HIP graph
- * and replays it (n_steps > 1). */
+ * step_key(key, step0 + s).  use_graph != 0 captures the steps between the
+ * first and the last in HIP graphs (eight steps per graph, plus a one-step
+ * graph for the remainder) and replays them; the graphs are kept until dt,
+ * the key mode or the key changes. */
 int gsqg_advance(gsqg_plan* plan, double dt, int n_steps, int key_mode, uint64_t key,
                  int64_t step0, int use_graph, void* stream);
 
