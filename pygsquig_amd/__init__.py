@@ -15,6 +15,7 @@ from .solver import gSQGSolver, gSQGSolverOptimized
 from .diagnostics import (compute_energy_flux, compute_energy_spectrum, compute_enstrophy,
                           compute_palinstrophy, compute_shell_spectrum, compute_total_energy)
 from .ensemble import EnsembleSolver
+from .closure import ClosedSolver, ClosureNet
 from ._lib import GsqgLibraryError
 
 __all__ = [
@@ -24,6 +25,6 @@ __all__ = [
     "compute_velocity_from_theta", "hyperviscosity", "gSQGSolver", "gSQGSolverOptimized",
     "compute_energy_spectrum", "compute_total_energy", "compute_enstrophy",
     "compute_palinstrophy", "compute_shell_spectrum", "compute_energy_flux", "EnsembleSolver",
-    "GsqgLibraryError",
+    "ClosureNet", "ClosedSolver", "GsqgLibraryError",
 ]
 __version__ = "0.1.0"

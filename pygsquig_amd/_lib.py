@@ -60,6 +60,20 @@ class GsqgPlanDesc(ctypes.Structure):
     ]
 
 
+class GsqgCnnBatch(ctypes.Structure):
+    _fields_ = [
+        ("theta", ctypes.c_void_p),
+        ("u", ctypes.c_void_p),
+        ("v", ctypes.c_void_p),
+        ("img_stride", ctypes.c_longlong),
+        ("sel_dev", ctypes.c_void_p),
+        ("batch", ctypes.c_int),
+        ("in_scale", ctypes.c_double * 3),
+        ("target", ctypes.c_void_p),
+        ("tgt_scale", ctypes.c_double),
+    ]
+
+
 # name -> (restype, argtypes)
 _VP = ctypes.c_void_p
 _SIGS = {
@@ -95,9 +109,28 @@ _SIGS = {
                                                  ctypes.POINTER(ctypes.c_double),
                                                  ctypes.POINTER(ctypes.c_double), _VP]),
     "gsqg_physical_diagnostics": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(ctypes.c_double), _VP]),
+    # include/gsqg_closure.h
+    "gsqg_cnn_num_params": (ctypes.c_int, []),
+    "gsqg_cnn_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.POINTER(_VP)]),
+    "gsqg_cnn_destroy": (ctypes.c_int, [_VP]),
+    "gsqg_cnn_device_bytes": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_size_t)]),
+    "gsqg_cnn_set_params": (ctypes.c_int, [_VP, _VP, _VP]),
+    "gsqg_cnn_get_params": (ctypes.c_int, [_VP, _VP, _VP]),
+    "gsqg_cnn_forward": (ctypes.c_int, [_VP, ctypes.POINTER(GsqgCnnBatch), ctypes.c_double, _VP,
+                                        _VP]),
+    "gsqg_cnn_gradient": (ctypes.c_int, [_VP, ctypes.POINTER(GsqgCnnBatch), _VP,
+                                         ctypes.POINTER(ctypes.c_double), _VP]),
+    "gsqg_cnn_train_step": (ctypes.c_int, [_VP, ctypes.POINTER(GsqgCnnBatch), ctypes.c_double,
+                                           ctypes.POINTER(ctypes.c_double), _VP]),
+    "gsqg_plan_attach_closure": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(ctypes.c_double),
+                                                ctypes.c_double]),
+    "gsqg_spectral_filter": (ctypes.c_int, [_VP, ctypes.c_int, _VP, ctypes.c_int, ctypes.c_double,
+                                            _VP, _VP, _VP]),
 }
 
-# Every symbol include/gsqg.h declares (checked by the CPU test suite).
+# Every symbol include/gsqg.h and include/gsqg_closure.h declare (checked by
+# the CPU test suite).
 EXPORTED = tuple(_SIGS)
 
 _lock = threading.Lock()

@@ -1,0 +1,677 @@
+// closure_kernels.hip -- forward, backward and Adam kernels of the learned
+// subgrid closure (BF16 weights and activations, FP32 accumulation).
+//
+// The 64 -> 64 convolutions are implicit GEMMs on the gfx950 bf16 MFMA
+// (v_mfma_f32_32x32x16_bf16): a block of 4 waves stages a 4 x 64 pixel tile
+// with its periodic halo and all nine taps of the layer's weights in LDS and
+// each wave accumulates one image row of 64 pixels x 64 output channels
+// (four 32 x 32 tiles).  The weight gradient is a GEMM whose reduction runs
+// over pixels: persistent blocks stage channel-major (transposed) copies of
+// the input, shifted by the three column offsets so every MFMA operand is
+// one aligned 16-byte LDS read, and keep all nine taps' accumulators in
+// registers across their tiles; per-block partials are summed in a fixed
+// order.  The 3-channel input layer and the 1-channel output layer carry
+// 2-3% of the arithmetic and run on FP32 FMA.
+#include <cmath>
+
+#include "closure_internal.h"
+#include "mfma_bf16.h"
+
+namespace cl {
+
+// ------------------------------------------------------------ helpers
+__device__ __forceinline__ float bf2f(uint32_t h) { return __uint_as_float(h << 16); }
+
+// float -> bfloat16, round to nearest, ties to even (quiet NaN kept).
+__device__ __forceinline__ uint32_t f2bf(float f) {
+  const uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (u >> 16) | 0x40u;
+  return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+
+__device__ __forceinline__ float rbf(float f) { return bf2f(f2bf(f)); }
+
+// GELU (erf form) and its derivative, float32.
+__device__ __forceinline__ float gelu(float z) {
+  return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f));
+}
+__device__ __forceinline__ float gelu_d(float z) {
+  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) +
+         z * expf(-0.5f * z * z) * 0.39894228040143268f;
+}
+
+__device__ __forceinline__ float unpack(const u32x4& v, int e) {
+  return bf2f((v[e >> 1] >> ((e & 1) * 16)) & 0xffffu);
+}
+
+__device__ __forceinline__ u32x4 pack8(const float (&f)[8]) {
+  u32x4 r;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) r[k] = f2bf(f[2 * k]) | (f2bf(f[2 * k + 1]) << 16);
+  return r;
+}
+
+// in[p] holds channels 0..7 of pixel p; out[c] gets pixels 0..7 of channel c.
+__device__ __forceinline__ void transpose8(const u32x4 (&in)[8], u32x4 (&out)[8]) {
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t lo = (in[2 * k][c >> 1] >> ((c & 1) * 16)) & 0xffffu;
+      const uint32_t hi = (in[2 * k + 1][c >> 1] >> ((c & 1) * 16)) & 0xffffu;
+      out[c][k] = lo | (hi << 16);
+    }
+  }
+}
+
+// Tile of the convolution kernels: TR image rows x TC columns, with a
+// one-pixel periodic halo; PS = bf16 stride of one pixel in LDS (144 bytes:
+// eight consecutive pixels' 16-byte reads fall in distinct bank groups).
+constexpr int TR = 4, TC = 64, HR = TR + 2, HC = TC + 2, PS = 72;
+constexpr int IN_E = HR * HC * PS;  // staged input tile (bf16 elements)
+constexpr int W_E = 9 * C * PS;     // staged weights
+constexpr int OS = 68;              // float stride of one pixel in the output staging
+
+__device__ __forceinline__ int wrap(int i, int n) { return i & (n - 1); }
+
+// Copy the (HR x HC)-pixel halo tile at (i0 - 1, j0 - 1) of image b into LDS.
+__device__ __forceinline__ void stage_tile(uint16_t* li, const uint16_t* __restrict__ src,
+                                           size_t ib, int i0, int j0, int H, int W, int tid) {
+  for (int e = tid; e < HR * HC * 8; e += 256) {
+    const int q = e >> 3, h = e & 7, r = q / HC, cc = q - r * HC;
+    const int gi = wrap(i0 - 1 + r, H), gj = wrap(j0 - 1 + cc, W);
+    *reinterpret_cast<u32x4*>(li + q * PS + 8 * h) =
+        *reinterpret_cast<const u32x4*>(src + ((ib + gi) * W + gj) * C + 8 * h);
+  }
+}
+
+// ------------------------------------------------------- layer 0 forward
+// z0 = bf16(conv(x, W0r) + b0), a0 = bf16(gelu(z0)), x = bf16(plane * scale).
+__global__ void __launch_bounds__(256) conv_in_fwd(Feat f, const float* __restrict__ W0r,
+                                                   const float* __restrict__ b0, uint16_t* z,
+                                                   uint16_t* a, int H, int W) {
+  __shared__ float xs[3][HR][HC];
+  __shared__ float ws[NW0 + C];
+  const int tid = threadIdx.x;
+  const int j0 = blockIdx.x * TC, i0 = blockIdx.y * TR, b = blockIdx.z;
+  const long long img = f.sel ? f.sel[b] : b;
+  for (int e = tid; e < 3 * HR * HC; e += 256) {
+    const int c = e / (HR * HC), q = e - c * (HR * HC), r = q / HC, cc = q - r * HC;
+    const int gi = wrap(i0 - 1 + r, H), gj = wrap(j0 - 1 + cc, W);
+    const double v = f.plane[c][img * f.img_stride + (long long)gi * W + gj] * f.in_scale[c];
+    xs[c][r][cc] = rbf((float)v);
+  }
+  for (int e = tid; e < NW0; e += 256) ws[e] = W0r[e];
+  if (tid < C) ws[NW0 + tid] = b0[tid];
+  __syncthreads();
+  const int r = tid >> 6, c = tid & 63;
+  float acc[C];
+#pragma unroll
+  for (int co = 0; co < C; ++co) acc[co] = 0.0f;
+  for (int t = 0; t < 9; ++t) {
+    const int dy = t / 3 - 1, dx = t % 3 - 1;
+#pragma unroll
+    for (int ci = 0; ci < 3; ++ci) {
+      const float x = xs[ci][r + 1 + dy][c + 1 + dx];
+      const float* w = ws + (t * 3 + ci) * C;
+#pragma unroll
+      for (int co = 0; co < C; ++co) acc[co] += x * w[co];
+    }
+  }
+  const size_t pix = ((size_t)b * H + i0 + r) * W + j0 + c;
+#pragma unroll
+  for (int h = 0; h < 8; ++h) {
+    float zf[8], af[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      zf[e] = rbf(acc[8 * h + e] + ws[NW0 + 8 * h + e]);
+      af[e] = gelu(zf[e]);
+    }
+    if (z) *reinterpret_cast<u32x4*>(z + pix * C + 8 * h) = pack8(zf);
+    *reinterpret_cast<u32x4*>(a + pix * C + 8 * h) = pack8(af);
+  }
+}
+
+// -------------------------------------------- 64 -> 64 layers (MFMA GEMM)
+// MODE 0 (forward): out[p][co] = sum_t sum_ci Wt[t][co][ci] in[p + off_t][ci];
+//   z = bf16(out + bias), a = bf16(gelu(z)).
+// MODE 1 (backward data): in = dZ of the layer, Wt = Wb (taps reversed, rows
+//   = input channels); out = bf16(out * gelu'(zprev)) is dZ of the layer below.
+template <int MODE>
+__global__ void __launch_bounds__(256) conv_mid(const uint16_t* __restrict__ in,
+                                                const uint16_t* __restrict__ Wt,
+                                                const float* __restrict__ bias,
+                                                const uint16_t* __restrict__ zprev, uint16_t* zout,
+                                                uint16_t* aout, int H, int W) {
+  __shared__ __attribute__((aligned(16))) uint16_t lds[IN_E + W_E];
+  uint16_t* li = lds;
+  uint16_t* lw = lds + IN_E;
+  const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
+  const int j0 = blockIdx.x * TC, i0 = blockIdx.y * TR, b = blockIdx.z;
+  const size_t ib = (size_t)b * H;
+  stage_tile(li, in, ib, i0, j0, H, W, tid);
+  for (int e = tid; e < 9 * C * 8; e += 256) {
+    const int row = e >> 3, h = e & 7;
+    *reinterpret_cast<u32x4*>(lw + row * PS + 8 * h) =
+        *reinterpret_cast<const u32x4*>(Wt + row * C + 8 * h);
+  }
+  __syncthreads();
+
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int g = 0; g < 2; ++g)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[h][g][r] = 0.0f;
+  const int lr = l & 31, lk = (l >> 5) * 8;
+  for (int t = 0; t < 9; ++t) {
+    const int dy = t / 3 - 1, dx = t % 3 - 1;
+    const uint16_t* wa = lw + (t * C + lr) * PS + lk;
+    const uint16_t* pb = li + ((wave + 1 + dy) * HC + 1 + dx + lr) * PS + lk;
+#pragma unroll
+    for (int kc = 0; kc < 4; ++kc) {
+      const u32x4 A0 = *reinterpret_cast<const u32x4*>(wa + 16 * kc);
+      const u32x4 A1 = *reinterpret_cast<const u32x4*>(wa + 32 * PS + 16 * kc);
+      const u32x4 B0 = *reinterpret_cast<const u32x4*>(pb + 16 * kc);
+      const u32x4 B1 = *reinterpret_cast<const u32x4*>(pb + 32 * PS + 16 * kc);
+      acc[0][0] = mfma_32x32x16(A0, B0, acc[0][0]);
+      acc[0][1] = mfma_32x32x16(A0, B1, acc[0][1]);
+      acc[1][0] = mfma_32x32x16(A1, B0, acc[1][0]);
+      acc[1][1] = mfma_32x32x16(A1, B1, acc[1][1]);
+    }
+  }
+  __syncthreads();  // every wave is done with the staged tile
+
+  // accumulators -> float staging [pixel][channel] -> coalesced stores
+  float* os = reinterpret_cast<float*>(lds);
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int g = 0; g < 2; ++g)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int co0 = 32 * h + 8 * q + 4 * (l >> 5);
+        const int px = wave * TC + 32 * g + lr;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          float v = acc[h][g][4 * q + e];
+          if (MODE == 0) v += bias[co0 + e];
+          os[px * OS + co0 + e] = v;
+        }
+      }
+  __syncthreads();
+  for (int e = tid; e < TR * TC * 8; e += 256) {
+    const int px = e >> 3, h = e & 7;
+    const size_t pix = (ib + i0 + (px >> 6)) * W + j0 + (px & 63);
+    float v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = os[px * OS + 8 * h + k];
+    if (MODE == 0) {
+      float zf[8], af[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        zf[k] = rbf(v[k]);
+        af[k] = gelu(zf[k]);
+      }
+      if (zout) *reinterpret_cast<u32x4*>(zout + pix * C + 8 * h) = pack8(zf);
+      *reinterpret_cast<u32x4*>(aout + pix * C + 8 * h) = pack8(af);
+    } else {
+      const u32x4 zp = *reinterpret_cast<const u32x4*>(zprev + pix * C + 8 * h);
+      float d[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) d[k] = v[k] * gelu_d(unpack(zp, k));
+      *reinterpret_cast<u32x4*>(aout + pix * C + 8 * h) = pack8(d);
+    }
+  }
+}
+
+// --------------------------------------------------- layer 5 forward
+// tau = out_scale * (conv(a4, W5r) + b5), one pixel per thread.
+__global__ void __launch_bounds__(256) conv_out_fwd(const uint16_t* __restrict__ a,
+                                                    const float* __restrict__ W5r,
+                                                    const float* __restrict__ P, float out_scale,
+                                                    float* tau, int H, int W) {
+  __shared__ __attribute__((aligned(16))) uint16_t li[IN_E];
+  __shared__ float ws[NW5];
+  const int tid = threadIdx.x;
+  const int j0 = blockIdx.x * TC, i0 = blockIdx.y * TR, b = blockIdx.z;
+  const size_t ib = (size_t)b * H;
+  stage_tile(li, a, ib, i0, j0, H, W, tid);
+  for (int e = tid; e < NW5; e += 256) ws[e] = W5r[e];
+  __syncthreads();
+  const int r = tid >> 6, c = tid & 63;
+  float acc = 0.0f;
+  for (int t = 0; t < 9; ++t) {
+    const int dy = t / 3 - 1, dx = t % 3 - 1;
+    const uint16_t* p = li + ((r + 1 + dy) * HC + c + 1 + dx) * PS;
+#pragma unroll
+    for (int h = 0; h < 8; ++h) {
+      const u32x4 v = *reinterpret_cast<const u32x4*>(p + 8 * h);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc += unpack(v, k) * ws[t * C + 8 * h + k];
+    }
+  }
+  tau[(ib + i0 + r) * W + j0 + c] = out_scale * (acc + P[OFF_B5]);
+}
+
+// ------------------------------------------------------------ loss
+// d = tau - float(target * tgt_scale); dtau = 2 d / n; per-block sum of d^2.
+__global__ void __launch_bounds__(256) loss_kernel(const float* __restrict__ tau,
+                                                   const double* __restrict__ tgt,
+                                                   long long img_stride, const int* sel,
+                                                   double tgt_scale, float* dtau, double* part,
+                                                   int HW, float n) {
+  __shared__ double red[256];
+  const int tid = threadIdx.x, b = blockIdx.y;
+  const int k = blockIdx.x * 256 + tid;
+  const long long img = sel ? sel[b] : b;
+  const size_t o = (size_t)b * HW + k;
+  const float tg = (float)(tgt[img * img_stride + k] * tgt_scale);
+  const float d = tau[o] - tg;
+  dtau[o] = (2.0f * d) / n;
+  red[tid] = (double)d * (double)d;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s) red[tid] += red[tid + s];
+    __syncthreads();
+  }
+  if (tid == 0) part[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+
+// ------------------------------------------------- layer 5 backward
+// dz4 = bf16(gelu'(z4) * sum_t W5r[t] dtau[p - off_t]); partial
+// dW5[t][ci] = sum_p a4[p + off_t][ci] dtau[p] and db5 = sum_p dtau[p] per
+// block (persistent over 4 x 64 tiles).
+__global__ void __launch_bounds__(256) conv_out_bwd(const uint16_t* __restrict__ a,
+                                                    const uint16_t* __restrict__ z,
+                                                    const float* __restrict__ dtau,
+                                                    const float* __restrict__ W5r, uint16_t* dz,
+                                                    float* part, int B, int H, int W) {
+  __shared__ __attribute__((aligned(16))) uint16_t li[IN_E];
+  __shared__ float ds[HR][HC];
+  __shared__ float ws[NW5];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < NW5; e += 256) ws[e] = W5r[e];
+  const int tx_n = W / TC, ty_n = H / TR, ntiles = B * tx_n * ty_n;
+  float gacc[3] = {0.0f, 0.0f, 0.0f};
+  const int r = tid >> 6, c = tid & 63;
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int tx = tile % tx_n, rest = tile / tx_n, ty = rest % ty_n, b = rest / ty_n;
+    const int i0 = ty * TR, j0 = tx * TC;
+    const size_t ib = (size_t)b * H;
+    __syncthreads();  // previous tile's LDS reads are done
+    stage_tile(li, a, ib, i0, j0, H, W, tid);
+    for (int e = tid; e < HR * HC; e += 256) {
+      const int rr = e / HC, cc = e - rr * HC;
+      ds[rr][cc] = dtau[(ib + wrap(i0 - 1 + rr, H)) * W + wrap(j0 - 1 + cc, W)];
+    }
+    __syncthreads();
+    const size_t pix = (ib + i0 + r) * W + j0 + c;
+    float d9[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) d9[t] = ds[r + 1 - (t / 3 - 1)][c + 1 - (t % 3 - 1)];
+#pragma unroll
+    for (int h = 0; h < 8; ++h) {
+      const u32x4 zp = *reinterpret_cast<const u32x4*>(z + pix * C + 8 * h);
+      float d[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        float s = 0.0f;
+#pragma unroll
+        for (int t = 0; t < 9; ++t) s += ws[t * C + 8 * h + k] * d9[t];
+        d[k] = s * gelu_d(unpack(zp, k));
+      }
+      *reinterpret_cast<u32x4*>(dz + pix * C + 8 * h) = pack8(d);
+    }
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+      const int o = tid + 256 * n;
+      if (o < NW5) {
+        const int t = o >> 6, ci = o & 63, dy = t / 3 - 1, dx = t % 3 - 1;
+        float s = 0.0f;
+        for (int rr = 0; rr < TR; ++rr)
+          for (int cc = 0; cc < TC; ++cc)
+            s += bf2f(li[((rr + 1 + dy) * HC + cc + 1 + dx) * PS + ci]) * ds[rr + 1][cc + 1];
+        gacc[n] += s;
+      } else if (o == NW5) {
+        float s = 0.0f;
+        for (int rr = 0; rr < TR; ++rr)
+          for (int cc = 0; cc < TC; ++cc) s += ds[rr + 1][cc + 1];
+        gacc[n] += s;
+      }
+    }
+  }
+#pragma unroll
+  for (int n = 0; n < 3; ++n) {
+    const int o = tid + 256 * n;
+    if (o < PART_OUT) part[(size_t)blockIdx.x * PART_OUT + o] = gacc[n];
+  }
+}
+
+// -------------------------------------- 64 -> 64 weight gradient (MFMA)
+// dW[t][ci][co] = sum_p a[p + off_t][ci] dz[p][co], db[co] = sum_p dz[p][co]:
+// a GEMM with M = ci, N = co and the reduction over pixels.  Persistent
+// blocks walk 2 x 64 tiles; wave w owns (ci half w & 1, co half w >> 1) of
+// all nine taps (144 accumulators).  LDS holds three channel-major copies of
+// the 4-row input halo, copy s shifted by s - 1 columns, and the
+// channel-major dz tile.
+constexpr int WG_AS = 4 * 64 + 8;    // bf16 per channel of an input copy (+16-byte pad)
+constexpr int WG_DS = 2 * 64 + 8;    // bf16 per channel of the dz tile
+constexpr int WG_A = 3 * C * WG_AS;  // three shifted copies
+constexpr int WG_D = C * WG_DS;
+
+__global__ void __launch_bounds__(256) conv_wgrad(const uint16_t* __restrict__ a,
+                                                  const uint16_t* __restrict__ dz, float* part,
+                                                  int B, int H, int W) {
+  __shared__ __attribute__((aligned(16))) uint16_t lds[WG_A + WG_D];
+  __shared__ float dbs[4][C];
+  uint16_t* la = lds;
+  uint16_t* ld = lds + WG_A;
+  const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
+  const int hc = wave & 1, ho = wave >> 1;
+  const int lr = l & 31, lk = (l >> 5) * 8;
+  const int tx_n = W / 64, ty_n = H / 2, ntiles = B * tx_n * ty_n;
+  f32x16 acc[9];
+#pragma unroll
+  for (int t = 0; t < 9; ++t)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[t][r] = 0.0f;
+  float dbacc = 0.0f;
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int tx = tile % tx_n, rest = tile / tx_n, ty = rest % ty_n, b = rest / ty_n;
+    const int i0 = ty * 2, j0 = tx * 64;
+    const size_t ib = (size_t)b * H;
+    // input copies: task (s, r, cg, m) = 8 pixels x 8 channels, transposed
+    for (int id = tid; id < 3 * 4 * 8 * 8; id += 256) {
+      const int m = id & 7, cg = (id >> 3) & 7, r = (id >> 6) & 3, s = id >> 8;
+      const int gi = wrap(i0 - 1 + r, H);
+      u32x4 vin[8], vout[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int gj = wrap(j0 + 8 * m + s - 1 + e, W);
+        vin[e] = *reinterpret_cast<const u32x4*>(a + ((ib + gi) * W + gj) * C + 8 * cg);
+      }
+      transpose8(vin, vout);
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        *reinterpret_cast<u32x4*>(la + (s * C + 8 * cg + e) * WG_AS + r * 64 + 8 * m) = vout[e];
+    }
+    if (tid < 2 * 8 * 8) {
+      const int m = tid & 7, cg = (tid >> 3) & 7, r = tid >> 6;
+      u32x4 vin[8], vout[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        vin[e] = *reinterpret_cast<const u32x4*>(dz + ((ib + i0 + r) * W + j0 + 8 * m + e) * C +
+                                                 8 * cg);
+      transpose8(vin, vout);
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        *reinterpret_cast<u32x4*>(ld + (8 * cg + e) * WG_DS + r * 64 + 8 * m) = vout[e];
+    }
+    __syncthreads();
+    for (int rr = 0; rr < 2; ++rr) {
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        const int px0 = 16 * ks + lk;
+        const u32x4 Bv =
+            *reinterpret_cast<const u32x4*>(ld + (32 * ho + lr) * WG_DS + rr * 64 + px0);
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+          const int dy = t / 3 - 1, dx = t % 3 - 1;
+          const u32x4 Av = *reinterpret_cast<const u32x4*>(
+              la + ((dx + 1) * C + 32 * hc + lr) * WG_AS + (rr + 1 + dy) * 64 + px0);
+          acc[t] = mfma_32x32x16(Av, Bv, acc[t]);
+        }
+      }
+    }
+    {
+      const int co = tid & 63, q = tid >> 6;
+      const uint16_t* dp = ld + co * WG_DS + (q >> 1) * 64 + (q & 1) * 32;
+      float s = 0.0f;
+      for (int k = 0; k < 32; ++k) s += bf2f(dp[k]);
+      dbacc += s;
+    }
+    __syncthreads();  // before the next tile overwrites LDS
+  }
+  float* pp = part + (size_t)blockIdx.x * PART_MID;
+#pragma unroll
+  for (int t = 0; t < 9; ++t)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int ci = 32 * hc + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3);
+      const int co = 32 * ho + lr;
+      pp[(t * C + ci) * C + co] = acc[t][r];
+    }
+  dbs[tid >> 6][tid & 63] = dbacc;
+  __syncthreads();
+  if (tid < C) pp[NWM + tid] = ((dbs[0][tid] + dbs[1][tid]) + dbs[2][tid]) + dbs[3][tid];
+}
+
+// ------------------------------------------------ layer 0 weight gradient
+// dW0[t][ci][co] = sum_p x[p + off_t][ci] dz0[p][co], db0 = sum_p dz0[p][co];
+// thread (co = tid & 63, group g = tid >> 6) owns the (t, ci) pairs
+// k = g, g + 4, ... < 27.  Persistent over 4 x 64 tiles.
+__global__ void __launch_bounds__(256) conv_in_wgrad(Feat f, const uint16_t* __restrict__ dz,
+                                                     float* part, int B, int H, int W) {
+  __shared__ float xs[3][HR][HC];
+  __shared__ __attribute__((aligned(16))) uint16_t ds[TR * TC * C];
+  const int tid = threadIdx.x, co = tid & 63, g = tid >> 6;
+  const int tx_n = W / TC, ty_n = H / TR, ntiles = B * tx_n * ty_n;
+  float acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  float accb = 0.0f;
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int tx = tile % tx_n, rest = tile / tx_n, ty = rest % ty_n, b = rest / ty_n;
+    const int i0 = ty * TR, j0 = tx * TC;
+    const long long img = f.sel ? f.sel[b] : b;
+    __syncthreads();
+    for (int e = tid; e < 3 * HR * HC; e += 256) {
+      const int c = e / (HR * HC), q = e - c * (HR * HC), r = q / HC, cc = q - r * HC;
+      const int gi = wrap(i0 - 1 + r, H), gj = wrap(j0 - 1 + cc, W);
+      const double v = f.plane[c][img * f.img_stride + (long long)gi * W + gj] * f.in_scale[c];
+      xs[c][r][cc] = rbf((float)v);
+    }
+    for (int e = tid; e < TR * TC * 8; e += 256) {
+      const int px = e >> 3, h = e & 7;
+      const size_t pix = ((size_t)b * H + i0 + (px >> 6)) * W + j0 + (px & 63);
+      *reinterpret_cast<u32x4*>(ds + px * C + 8 * h) =
+          *reinterpret_cast<const u32x4*>(dz + pix * C + 8 * h);
+    }
+    __syncthreads();
+    for (int px = 0; px < TR * TC; ++px) {
+      const int r = px >> 6, c = px & 63;
+      const float d = bf2f(ds[px * C + co]);
+#pragma unroll
+      for (int n = 0; n < 7; ++n) {
+        const int k = g + 4 * n;
+        if (k < 27) {
+          const int t = k / 3, ci = k - 3 * t;
+          acc[n] += xs[ci][r + 1 + t / 3 - 1][c + 1 + t % 3 - 1] * d;
+        }
+      }
+      if (g == 0) accb += d;
+    }
+  }
+  float* pp = part + (size_t)blockIdx.x * PART_IN;
+#pragma unroll
+  for (int n = 0; n < 7; ++n) {
+    const int k = g + 4 * n;
+    if (k < 27) pp[k * C + co] = acc[n];
+  }
+  if (g == 0) pp[NW0 + co] = accb;
+}
+
+// --------------------------------------------------- partial reduction
+// out[o] = sum over blocks b of part[b * n + o]: four slices of the blocks
+// per output, summed in double and combined in a fixed order.
+__global__ void __launch_bounds__(256) reduce_parts(const float* __restrict__ part, int nb, int n,
+                                                    float* out) {
+  __shared__ double red[4][64];
+  const int tid = threadIdx.x, o = blockIdx.x * 64 + (tid & 63), sl = tid >> 6;
+  double s = 0.0;
+  if (o < n)
+    for (int b = sl; b < nb; b += 4) s += part[(size_t)b * n + o];
+  red[sl][tid & 63] = s;
+  __syncthreads();
+  if (sl == 0 && o < n)
+    out[o] = (float)(((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid]);
+}
+
+// ------------------------------------------------------------ optimiser
+__global__ void __launch_bounds__(256) adam_kernel(float* P, const float* __restrict__ g,
+                                                   float* m, float* v, float lr, float b1,
+                                                   float omb1, float b2, float omb2, float eps,
+                                                   float c1, float c2, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + omb1 * gi;
+  const float vi = b2 * v[i] + omb2 * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  P[i] = P[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// Derived weights: bf16 forward (rows = output channel) and backward-data
+// (taps reversed, rows = input channel) layouts of layers 1-4, rounded
+// copies of W0 and W5.
+__global__ void __launch_bounds__(256) prepare_kernel(const float* __restrict__ P, uint16_t* Wf,
+                                                      uint16_t* Wb, float* W0r, float* W5r) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < 4 * NWM) {
+    const int l = i / NWM, o = i - l * NWM;
+    const int t = o / (C * C), ci = (o / C) % C, co = o % C;
+    const uint16_t h = (uint16_t)f2bf(P[off_w(l + 1) + o]);
+    Wf[(size_t)l * NWM + (t * C + co) * C + ci] = h;
+    Wb[(size_t)l * NWM + ((8 - t) * C + ci) * C + co] = h;
+  } else if (i < 4 * NWM + NW0) {
+    const int k = i - 4 * NWM;
+    W0r[k] = rbf(P[OFF_W0 + k]);
+  } else if (i < 4 * NWM + NW0 + NW5) {
+    const int k = i - 4 * NWM - NW0;
+    W5r[k] = rbf(P[OFF_W5 + k]);
+  }
+}
+
+// ----------------------------------------------------- spectral filter
+// Coarse (nc x nc) spectrum of a fine (nf x nf) one, both full complex128
+// [kx][ky]: out[i][j] = mask[i][j] ? scale * in[fi][fj] + add[i][j] : 0,
+// with fi = i for i < nc/2 and nf - nc + i above (same wavenumber).
+__global__ void __launch_bounds__(256) spectral_filter_kernel(const double2* __restrict__ in,
+                                                              int nf, double2* out, int nc,
+                                                              double scale,
+                                                              const uint8_t* __restrict__ mask,
+                                                              const double2* __restrict__ add) {
+  const int j = blockIdx.x * 256 + threadIdx.x, i = blockIdx.y;
+  if (j >= nc) return;
+  const size_t o = (size_t)i * nc + j;
+  double2 r;
+  r.x = 0.0;
+  r.y = 0.0;
+  if (mask[o]) {
+    const int fi = i < nc / 2 ? i : nf - nc + i, fj = j < nc / 2 ? j : nf - nc + j;
+    const double2 x = in[(size_t)fi * nf + fj];
+    r.x = scale * x.x;
+    r.y = scale * x.y;
+    if (add) {
+      r.x += add[o].x;
+      r.y += add[o].y;
+    }
+  }
+  out[o] = r;
+}
+
+// ------------------------------------------------------------ launchers
+hipError_t launch_spectral_filter(const double2* in, int nf, double2* out, int nc, double scale,
+                                  const uint8_t* mask, const double2* add, hipStream_t st) {
+  hipLaunchKernelGGL(spectral_filter_kernel, dim3((nc + 255) / 256, nc), dim3(256), 0, st, in, nf,
+                     out, nc, scale, mask, add);
+  return hipGetLastError();
+}
+
+static const int kPersist = 512;  // blocks of the persistent gradient kernels
+
+static int persist_blocks(int ntiles) { return ntiles < kPersist ? ntiles : kPersist; }
+
+size_t part_floats(int B, int H, int W) {
+  const int t4 = B * (H / TR) * (W / TC);
+  size_t a = (size_t)persist_blocks(t4) * PART_OUT;
+  size_t b = (size_t)kPersist * PART_MID;
+  size_t c = (size_t)persist_blocks(t4) * PART_IN;
+  return a > b ? (a > c ? a : c) : (b > c ? b : c);
+}
+
+static void reduce(const float* part, int nb, int n, float* out, hipStream_t st) {
+  hipLaunchKernelGGL(reduce_parts, dim3((n + 63) / 64), dim3(256), 0, st, part, nb, n, out);
+}
+
+hipError_t launch_forward(const Net& net, const Feat& f, int B, int H, int W, uint16_t* const z[5],
+                          uint16_t* const a[5], float* tau, float out_scale, hipStream_t st) {
+  const dim3 g(W / TC, H / TR, B);
+  hipLaunchKernelGGL(conv_in_fwd, g, dim3(256), 0, st, f, net.W0r, net.P + OFF_B0, z[0], a[0], H,
+                     W);
+  for (int l = 1; l <= 4; ++l)
+    hipLaunchKernelGGL(conv_mid<0>, g, dim3(256), 0, st, (const uint16_t*)a[l - 1],
+                       net.Wf + (size_t)(l - 1) * NWM, net.P + off_b(l), (const uint16_t*)nullptr,
+                       z[l], a[l], H, W);
+  hipLaunchKernelGGL(conv_out_fwd, g, dim3(256), 0, st, (const uint16_t*)a[4], net.W5r, net.P,
+                     out_scale, tau, H, W);
+  return hipGetLastError();
+}
+
+hipError_t launch_loss(const float* tau, const double* tgt, long long img_stride, const int* sel,
+                       double tgt_scale, float* dtau, double* part, int B, int H, int W,
+                       hipStream_t st) {
+  const int HW = H * W;
+  hipLaunchKernelGGL(loss_kernel, dim3(HW / 256, B), dim3(256), 0, st, tau, tgt, img_stride, sel,
+                     tgt_scale, dtau, part, HW, (float)((double)B * HW));
+  return hipGetLastError();
+}
+
+hipError_t launch_backward(const Net& net, const Feat& f, int B, int H, int W,
+                           uint16_t* const z[5], uint16_t* const a[5], const float* dtau,
+                           uint16_t* dz0, uint16_t* dz1, float* part, float* grad,
+                           hipStream_t st) {
+  const dim3 g(W / TC, H / TR, B);
+  const int t4 = B * (H / TR) * (W / TC), t2 = B * (H / 2) * (W / 64);
+  const int nb4 = persist_blocks(t4), nb2 = t2 < kPersist ? t2 : kPersist;
+  uint16_t* cur = dz0;
+  uint16_t* nxt = dz1;
+  hipLaunchKernelGGL(conv_out_bwd, dim3(nb4), dim3(256), 0, st, (const uint16_t*)a[4],
+                     (const uint16_t*)z[4], dtau, net.W5r, cur, part, B, H, W);
+  reduce(part, nb4, PART_OUT, grad + OFF_W5, st);
+  for (int l = 4; l >= 1; --l) {
+    hipLaunchKernelGGL(conv_wgrad, dim3(nb2), dim3(256), 0, st, (const uint16_t*)a[l - 1],
+                       (const uint16_t*)cur, part, B, H, W);
+    reduce(part, nb2, PART_MID, grad + off_w(l), st);
+    hipLaunchKernelGGL(conv_mid<1>, g, dim3(256), 0, st, (const uint16_t*)cur,
+                       net.Wb + (size_t)(l - 1) * NWM, (const float*)nullptr,
+                       (const uint16_t*)z[l - 1], (uint16_t*)nullptr, nxt, H, W);
+    uint16_t* s = cur;
+    cur = nxt;
+    nxt = s;
+  }
+  hipLaunchKernelGGL(conv_in_wgrad, dim3(nb4), dim3(256), 0, st, f, (const uint16_t*)cur, part, B,
+                     H, W);
+  reduce(part, nb4, PART_IN, grad + OFF_W0, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_adam(float* P, const float* grad, float* m, float* v, double lr, double b1,
+                       double b2, double eps, int t, hipStream_t st) {
+  // float32 constants as float32(value computed in double)
+  const float c1 = (float)(1.0 - std::pow(b1, (double)t)), c2 = (float)(1.0 - std::pow(b2, (double)t));
+  hipLaunchKernelGGL(adam_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, st, P, grad, m, v,
+                     (float)lr, (float)b1, (float)(1.0 - b1), (float)b2, (float)(1.0 - b2),
+                     (float)eps, c1, c2, NPARAM);
+  return hipGetLastError();
+}
+
+hipError_t launch_prepare(const float* P, uint16_t* Wf, uint16_t* Wb, float* W0r, float* W5r,
+                          hipStream_t st) {
+  const int n = 4 * NWM + NW0 + NW5;
+  hipLaunchKernelGGL(prepare_kernel, dim3((n + 255) / 256), dim3(256), 0, st, P, Wf, Wb, W0r, W5r);
+  return hipGetLastError();
+}
+
+}  // namespace cl

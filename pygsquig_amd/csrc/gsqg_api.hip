@@ -8,6 +8,8 @@
 #include <vector>
 
 #include "../../include/gsqg.h"
+#include "../../include/gsqg_closure.h"
+#include "closure_internal.h"
 #include "gsqg_internal.h"
 
 namespace gsqg {
@@ -114,6 +116,16 @@ struct gsqg_plan {
   double g_dt = 0.0;
   int g_key_mode = -1;
   uint64_t g_key = 0;
+  // learned closure (gsqg_plan_attach_closure): the network, its input and
+  // output scales, the stage state, its theta (mixed and physical), u, v,
+  // two activation buffers and the flux tau
+  const gsqg_cnn* cl_net = nullptr;
+  double cl_in[3] = {1.0, 1.0, 1.0};
+  double cl_out = 1.0;
+  double2 *cl_ys = nullptr, *cl_f = nullptr;
+  double *cl_th = nullptr, *cl_u = nullptr, *cl_v = nullptr;
+  uint16_t *cl_a0 = nullptr, *cl_a1 = nullptr;
+  float* cl_tau = nullptr;
 };
 
 template <typename T>
@@ -130,7 +142,8 @@ static void free_plan(gsqg_plan* p) {
   void* ptrs[] = {p->th,   p->th2,   p->acc,  p->ys,   p->J,     p->F[0], p->F[1], p->F[2],
                   p->F[3], p->s_th,  p->s_acc, p->s_ys, p->s_J,  p->s_F0, p->s_F1, p->uphys,
                   p->vphys, p->k1d,  p->flags, p->tw,   p->ctrl, p->part, p->ring, p->a_tab, p->a_val,
-                  p->a_tmp, p->norms, p->d_part, p->d_out, p->d_edges, p->d_phys, p->malpha};
+                  p->a_tmp, p->norms, p->d_part, p->d_out, p->d_edges, p->d_phys, p->malpha,
+                  p->cl_ys, p->cl_f, p->cl_th, p->cl_u, p->cl_v, p->cl_a0, p->cl_a1, p->cl_tau};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (hipEvent_t e : p->ev_pool) (void)hipEventDestroy(e);
@@ -331,7 +344,9 @@ static hipError_t ck(gsqg_plan* p, int job, const CKArgs& a, int ny, hipStream_t
 }
 
 static hipError_t rk(gsqg_plan* p, int job, const RKArgs& a, int ny, hipStream_t st) {
-  const int cls = job == RK_THETA ? KC_THETA_ROW : job == RK_SCALAR ? KC_SCALAR_ROW : KC_OTHER;
+  const int cls = (job == RK_THETA || job == RK_THETA_CL) ? KC_THETA_ROW
+                  : job == RK_SCALAR                         ? KC_SCALAR_ROW
+                                                             : KC_OTHER;
   if (!p->prof) return launch_rk(p->N, job, a, ny, st);
   LaunchEv ev{};
   hipError_t r = prof_pair(p, cls, &ev);
@@ -365,6 +380,7 @@ static CKArgs ck_theta(gsqg_plan* p, double2* th, int e, int prologue, int rate_
   a.a_val = p->a_val;
   a.n_a = p->n_ring;
   a.field_stride = 0;
+  a.ystage = p->cl_net ? p->cl_ys : nullptr;
   if (th == p->th && p->n_mem > 1) {
     a.field_stride = (long long)p->hsz;
     a.member_alpha = p->malpha;
@@ -388,6 +404,56 @@ static RKArgs rk_theta(gsqg_plan* p, int red_slot, int inc_step, bool uv_out) {
   a.eps = p->ph.eps;
   a.field_stride = (long long)p->hsz;  // ensemble members; 0 .. n_mem - 1 as y-slices
   return a;
+}
+
+// Closure flux of the current stage: theta of the stage state (stored by the
+// previous column prologue), u and v from the prologue's velocity fields,
+// then the network.
+static int closure_tau(gsqg_plan* p, hipStream_t st) {
+  const int N = p->N;
+  CKArgs a = ck_theta(p, p->cl_ys, 0, 0, 0, 0, 0.0);
+  a.F[0] = p->cl_f;
+  a.ystage = nullptr;
+  HIPCHK(ck(p, CK_INV, a, 1, st));
+  RKArgs r{};
+  r.F[0] = p->cl_f;
+  r.phys_out = p->cl_th;
+  r.tw = p->tw;
+  r.ctrl = p->ctrl;
+  HIPCHK(rk(p, RK_C2R, r, 1, st));
+  RKArgs v{};
+  v.F[0] = p->F[2];
+  v.F[1] = p->F[3];
+  v.uout = p->cl_u;
+  v.vout = p->cl_v;
+  v.tw = p->tw;
+  v.ctrl = p->ctrl;
+  HIPCHK(rk(p, RK_VEL, v, 1, st));
+  cl::Feat f{};
+  f.plane[0] = p->cl_th;
+  f.plane[1] = p->cl_u;
+  f.plane[2] = p->cl_v;
+  f.img_stride = 0;
+  f.sel = nullptr;
+  for (int c = 0; c < 3; ++c) f.in_scale[c] = p->cl_in[c];
+  uint16_t* const z[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  uint16_t* const act[5] = {p->cl_a0, p->cl_a1, p->cl_a0, p->cl_a1, p->cl_a0};
+  HIPCHK(cl::launch_forward(cnn_net(p->cl_net), f, 1, N, N, z, act, p->cl_tau, (float)p->cl_out,
+                            st));
+  return GSQG_OK;
+}
+
+// Row pass of a theta stage (with the closure flux when one is attached).
+static int theta_row(gsqg_plan* p, RKArgs a, int ny, hipStream_t st) {
+  if (p->cl_net) {
+    int rc = closure_tau(p, st);
+    if (rc) return rc;
+    a.tau = p->cl_tau;
+    HIPCHK(rk(p, RK_THETA_CL, a, 1, st));
+    return GSQG_OK;
+  }
+  HIPCHK(rk(p, RK_THETA, a, ny, st));
+  return GSQG_OK;
 }
 
 // The scalar kernels read kappa from ph.kappa; scalars with different kappa
@@ -466,7 +532,8 @@ static int theta_rest(gsqg_plan* p, double dt, bool derived, bool last_prologue,
   const int N = p->N;
   for (int e = 1; e <= 3; ++e) {
     HIPCHK(ck(p, CK_THETA, ck_theta(p, p->th, e, 1, e + 1, 0, dt), p->n_mem, st));
-    HIPCHK(rk(p, RK_THETA, rk_theta(p, e + 1, 0, false), p->n_mem, st));
+    int rc = theta_row(p, rk_theta(p, e + 1, 0, false), p->n_mem, st);
+    if (rc) return rc;
   }
   HIPCHK(ck(p, CK_THETA,
                    ck_theta(p, p->th, 4, last_prologue ? 1 : 0, last_rate, derived ? 1 : 0, dt),
@@ -485,8 +552,8 @@ static int theta_rest(gsqg_plan* p, double dt, bool derived, bool last_prologue,
 static int step_body(gsqg_plan* p, double dt, bool derived, bool scalars_pending,
                      bool last_prologue, int last_rate, bool scalar_next_prologue,
                      hipStream_t st, hipStream_t side = nullptr) {
-  HIPCHK(rk(p, RK_THETA, rk_theta(p, 1, derived ? 1 : 0, scalars_pending), p->n_mem, st));
-  int rc;
+  int rc = theta_row(p, rk_theta(p, 1, derived ? 1 : 0, scalars_pending), p->n_mem, st);
+  if (rc) return rc;
   if (scalars_pending && side) {
     HIPCHK(hipEventRecord(p->ev_fork, st));
     HIPCHK(hipStreamWaitEvent(side, p->ev_fork, 0));
@@ -539,7 +606,7 @@ extern "C" int gsqg_advance(gsqg_plan* p, double dt, int n_steps, int key_mode, 
   // step 0
   {
     const bool last = (n_steps == 1);
-    HIPCHK(rk(p, RK_THETA, rk_theta(p, 1, derived ? 1 : 0, false), p->n_mem, st));
+    if ((rc = theta_row(p, rk_theta(p, 1, derived ? 1 : 0, false), p->n_mem, st))) return rc;
     if ((rc = theta_rest(p, dt, derived, !last || sc, last ? 0 : 1, st))) return rc;
   }
   // middle steps 1 .. n-2
@@ -679,7 +746,8 @@ extern "C" int gsqg_rhs(gsqg_plan* p, const void* X, void* out, uint64_t key, vo
   CKArgs c0 = ck_theta(p, p->th2, 0, 1, 1, 0, 0.0);
   c0.ph.scheme = 0;
   HIPCHK(ck(p, CK_THETA, c0, 1, st));
-  HIPCHK(rk(p, RK_THETA, rk_theta(p, 1, 0, false), 1, st));
+  int rc = theta_row(p, rk_theta(p, 1, 0, false), 1, st);
+  if (rc) return rc;
   // Stage-1 epilogue writes k1 into acc (th2 is left untouched for e == 1).
   CKArgs c1 = ck_theta(p, p->th2, 1, 0, 0, 0, 0.0);
   c1.ph.scheme = 0;
@@ -882,5 +950,227 @@ extern "C" int gsqg_physical_diagnostics(gsqg_plan* p, const void* X, double* ou
   out[0] = 0.5 * (h[1] * inv);
   out[1] = 0.5 * (h[0] * inv);
   out[2] = h[2];
+  return GSQG_OK;
+}
+
+// ------------------------------------------------------------ closure
+// Host side of include/gsqg_closure.h.
+
+template <typename T>
+static hipError_t cnn_alloc(gsqg_cnn* n, T** ptr, size_t count) {
+  size_t b = count * sizeof(T);
+  if (b == 0) b = 16;
+  hipError_t e = hipMalloc((void**)ptr, b);
+  if (e == hipSuccess) n->bytes += b;
+  return e;
+}
+
+static void free_cnn(gsqg_cnn* n) {
+  if (!n) return;
+  void* ptrs[] = {n->P,    n->G,    n->M,    n->V,    n->Wf,   n->Wb,   n->W0r,  n->W5r,
+                  n->z[0], n->z[1], n->z[2], n->z[3], n->z[4], n->a[0], n->a[1], n->a[2],
+                  n->a[3], n->a[4], n->dz0,  n->dz1,  n->tau,  n->dtau, n->part, n->lpart};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  delete n;
+}
+
+extern "C" int gsqg_cnn_num_params(void) { return cl::NPARAM; }
+
+extern "C" int gsqg_cnn_create(int h, int w, int max_batch, gsqg_cnn** out) {
+  if (!out) return fail(GSQG_ERR_ARG, "null output handle");
+  *out = nullptr;
+  if (!is_pow2(h) || !is_pow2(w) || w < 64 || h < 4)
+    return fail(GSQG_ERR_ARG, "image sides must be powers of two with w >= 64 and h >= 4, got " +
+                                  std::to_string(h) + " x " + std::to_string(w));
+  if (max_batch < 1) return fail(GSQG_ERR_ARG, "max_batch must be >= 1");
+  gsqg_cnn* n = new gsqg_cnn();
+  n->H = h;
+  n->W = w;
+  n->B = max_batch;
+  const size_t px = (size_t)max_batch * h * w;
+  hipError_t e = hipSuccess;
+  auto A = [&](auto** p, size_t c) {
+    if (e == hipSuccess) e = cnn_alloc(n, p, c);
+  };
+  A(&n->P, cl::NPARAM);
+  A(&n->G, cl::NPARAM);
+  A(&n->M, cl::NPARAM);
+  A(&n->V, cl::NPARAM);
+  A(&n->Wf, 4 * (size_t)cl::NWM);
+  A(&n->Wb, 4 * (size_t)cl::NWM);
+  A(&n->W0r, cl::NW0);
+  A(&n->W5r, cl::NW5);
+  for (int l = 0; l < 5; ++l) {
+    A(&n->z[l], px * cl::C);
+    A(&n->a[l], px * cl::C);
+  }
+  A(&n->dz0, px * cl::C);
+  A(&n->dz1, px * cl::C);
+  A(&n->tau, px);
+  A(&n->dtau, px);
+  A(&n->part, cl::part_floats(max_batch, h, w));
+  A(&n->lpart, px / 256);
+  if (e == hipSuccess) e = hipMemset(n->P, 0, cl::NPARAM * sizeof(float));
+  if (e == hipSuccess) e = hipMemset(n->M, 0, cl::NPARAM * sizeof(float));
+  if (e == hipSuccess) e = hipMemset(n->V, 0, cl::NPARAM * sizeof(float));
+  if (e == hipSuccess) e = cl::launch_prepare(n->P, n->Wf, n->Wb, n->W0r, n->W5r, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    free_cnn(n);
+    return fail(GSQG_ERR_HIP, std::string("network allocation: ") + hipGetErrorString(e));
+  }
+  *out = n;
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_cnn_destroy(gsqg_cnn* n) {
+  free_cnn(n);
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_cnn_device_bytes(const gsqg_cnn* n, size_t* bytes) {
+  if (!n || !bytes) return fail(GSQG_ERR_ARG, "null argument");
+  *bytes = n->bytes;
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_cnn_set_params(gsqg_cnn* n, const float* params, void* stream) {
+  if (!n || !params) return fail(GSQG_ERR_ARG, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(hipMemcpyAsync(n->P, params, cl::NPARAM * sizeof(float), hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemsetAsync(n->M, 0, cl::NPARAM * sizeof(float), st));
+  HIPCHK(hipMemsetAsync(n->V, 0, cl::NPARAM * sizeof(float), st));
+  n->t = 0;
+  HIPCHK(cl::launch_prepare(n->P, n->Wf, n->Wb, n->W0r, n->W5r, st));
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_cnn_get_params(const gsqg_cnn* n, float* params, void* stream) {
+  if (!n || !params) return fail(GSQG_ERR_ARG, "null argument");
+  HIPCHK(hipMemcpyAsync(params, n->P, cl::NPARAM * sizeof(float), hipMemcpyDeviceToDevice,
+                        (hipStream_t)stream));
+  return GSQG_OK;
+}
+
+static int cnn_feat(const gsqg_cnn* n, const gsqg_cnn_batch* b, bool need_target, cl::Feat* f) {
+  if (!n || !b) return fail(GSQG_ERR_ARG, "null argument");
+  if (!b->theta || !b->u || !b->v) return fail(GSQG_ERR_ARG, "null input plane");
+  if (b->batch < 1 || b->batch > n->B)
+    return fail(GSQG_ERR_ARG, "batch must be in [1, " + std::to_string(n->B) + "], got " +
+                                  std::to_string(b->batch));
+  if (need_target && !b->target) return fail(GSQG_ERR_ARG, "null target");
+  f->plane[0] = b->theta;
+  f->plane[1] = b->u;
+  f->plane[2] = b->v;
+  f->img_stride = b->img_stride;
+  f->sel = b->sel_dev;
+  for (int c = 0; c < 3; ++c) f->in_scale[c] = b->in_scale[c];
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_cnn_forward(gsqg_cnn* n, const gsqg_cnn_batch* b, double out_scale, float* tau,
+                                void* stream) {
+  cl::Feat f{};
+  int rc = cnn_feat(n, b, false, &f);
+  if (rc) return rc;
+  if (!tau) return fail(GSQG_ERR_ARG, "null output");
+  uint16_t* const z[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  HIPCHK(cl::launch_forward(cnn_net(n), f, b->batch, n->H, n->W, z, n->a, tau, (float)out_scale,
+                            (hipStream_t)stream));
+  return GSQG_OK;
+}
+
+// Forward (keeping z and a), loss, backward into n->G.
+static int cnn_gradient(gsqg_cnn* n, const gsqg_cnn_batch* b, hipStream_t st) {
+  cl::Feat f{};
+  int rc = cnn_feat(n, b, true, &f);
+  if (rc) return rc;
+  const int B = b->batch;
+  HIPCHK(cl::launch_forward(cnn_net(n), f, B, n->H, n->W, n->z, n->a, n->tau, 1.0f, st));
+  HIPCHK(cl::launch_loss(n->tau, b->target, b->img_stride, b->sel_dev, b->tgt_scale, n->dtau,
+                         n->lpart, B, n->H, n->W, st));
+  HIPCHK(cl::launch_backward(cnn_net(n), f, B, n->H, n->W, n->z, n->a, n->dtau, n->dz0, n->dz1,
+                             n->part, n->G, st));
+  return GSQG_OK;
+}
+
+static int cnn_loss(const gsqg_cnn* n, int B, hipStream_t st, double* loss) {
+  const size_t nb = (size_t)B * n->H * n->W / 256;
+  std::vector<double> h(nb);
+  HIPCHK(hipMemcpyAsync(h.data(), n->lpart, nb * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  double s = 0.0;
+  for (double x : h) s += x;
+  *loss = s / ((double)B * n->H * n->W);
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_cnn_gradient(gsqg_cnn* n, const gsqg_cnn_batch* b, float* grad, double* loss,
+                                 void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  int rc = cnn_gradient(n, b, st);
+  if (rc) return rc;
+  if (grad)
+    HIPCHK(hipMemcpyAsync(grad, n->G, cl::NPARAM * sizeof(float), hipMemcpyDeviceToDevice, st));
+  return loss ? cnn_loss(n, b->batch, st, loss) : GSQG_OK;
+}
+
+extern "C" int gsqg_cnn_train_step(gsqg_cnn* n, const gsqg_cnn_batch* b, double lr, double* loss,
+                                   void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (!(lr > 0.0)) return fail(GSQG_ERR_ARG, "learning rate must be positive");
+  int rc = cnn_gradient(n, b, st);
+  if (rc) return rc;
+  n->t += 1;
+  HIPCHK(cl::launch_adam(n->P, n->G, n->M, n->V, lr, 0.9, 0.999, 1e-8, n->t, st));
+  HIPCHK(cl::launch_prepare(n->P, n->Wf, n->Wb, n->W0r, n->W5r, st));
+  return loss ? cnn_loss(n, b->batch, st, loss) : GSQG_OK;
+}
+
+extern "C" int gsqg_plan_attach_closure(gsqg_plan* p, const gsqg_cnn* net, const double* in_scale,
+                                        double out_scale) {
+  if (!p) return fail(GSQG_ERR_ARG, "null plan");
+  // captured graphs hold the previous schedule
+  if (p->gexec) HIPCHK(hipGraphExecDestroy(p->gexec));
+  if (p->gexec_k) HIPCHK(hipGraphExecDestroy(p->gexec_k));
+  p->gexec = p->gexec_k = nullptr;
+  p->g_key_mode = -1;
+  if (!net) {
+    p->cl_net = nullptr;
+    return GSQG_OK;
+  }
+  if (!in_scale) return fail(GSQG_ERR_ARG, "null input scales");
+  const int N = p->N;
+  if (net->H != N || net->W != N)
+    return fail(GSQG_ERR_ARG, "the network is " + std::to_string(net->H) + " x " +
+                                  std::to_string(net->W) + ", the plan's grid " +
+                                  std::to_string(N) + " x " + std::to_string(N));
+  if (p->n_mem > 1) return fail(GSQG_ERR_UNSUPPORTED, "closure on an ensemble plan");
+  if (!p->cl_ys) {
+    const size_t n2 = (size_t)N * N;
+    HIPCHK(dalloc(p, &p->cl_ys, p->hsz));
+    HIPCHK(dalloc(p, &p->cl_f, p->hsz));
+    HIPCHK(dalloc(p, &p->cl_th, n2));
+    HIPCHK(dalloc(p, &p->cl_u, n2));
+    HIPCHK(dalloc(p, &p->cl_v, n2));
+    HIPCHK(dalloc(p, &p->cl_a0, n2 * cl::C));
+    HIPCHK(dalloc(p, &p->cl_a1, n2 * cl::C));
+    HIPCHK(dalloc(p, &p->cl_tau, n2));
+  }
+  p->cl_net = net;
+  for (int c = 0; c < 3; ++c) p->cl_in[c] = in_scale[c];
+  p->cl_out = out_scale;
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_spectral_filter(const void* in, int nf, void* out, int nc, double scale,
+                                    const uint8_t* mask, const void* add, void* stream) {
+  if (!in || !out || !mask) return fail(GSQG_ERR_ARG, "null argument");
+  if (nf < 2 || nc < 2 || (nf & 1) || (nc & 1) || nc > nf)
+    return fail(GSQG_ERR_ARG, "need even sizes with nc <= nf, got nf=" + std::to_string(nf) +
+                                  " nc=" + std::to_string(nc));
+  HIPCHK(cl::launch_spectral_filter((const double2*)in, nf, (double2*)out, nc, scale, mask,
+                                    (const double2*)add, (hipStream_t)stream));
   return GSQG_OK;
 }

@@ -63,7 +63,9 @@ struct ATab {
 };
 
 enum CKJob { CK_THETA = 0, CK_SCALAR = 1, CK_VEL = 2, CK_FWD = 3, CK_INV = 4 };
-enum RKJob { RK_THETA = 0, RK_SCALAR = 1, RK_VEL = 2, RK_R2C = 3, RK_C2R = 4 };
+// RK_THETA_CL: RK_THETA with a physical subgrid flux tau added to the
+// Jacobian before its forward transform (learned closure).
+enum RKJob { RK_THETA = 0, RK_SCALAR = 1, RK_VEL = 2, RK_R2C = 3, RK_C2R = 4, RK_THETA_CL = 5 };
 
 // Optional start/stop events recorded at the kernel's own boundaries
 // (hipExtLaunchKernelGGL), for live per-kernel timing.
@@ -94,6 +96,7 @@ struct CKArgs {
   double2* a_val;
   int n_a;
   long long field_stride; // elements between consecutive scalars (blockIdx.y)
+  double2* ystage;       // prologue state y written here when non-null (closure features)
 };
 
 struct RKArgs {
@@ -114,6 +117,7 @@ struct RKArgs {
   int inc_step;          // block 0 increments ctrl->step
   double eps;
   long long field_stride;
+  const float* tau;      // RK_THETA_CL: physical (N, N) flux added to J
 };
 
 }  // namespace gsqg

@@ -56,7 +56,8 @@ struct RKShape {
 // Spectral fields a row-kernel job reads.
 template <int JOB>
 struct RKFields {
-  static constexpr int value = JOB == RK_THETA ? 4 : ((JOB == RK_SCALAR || JOB == RK_VEL) ? 2 : 1);
+  static constexpr int value =
+      (JOB == RK_THETA || JOB == RK_THETA_CL) ? 4 : ((JOB == RK_SCALAR || JOB == RK_VEL) ? 2 : 1);
 };
 
 // ------------------------------------------------------- per-mode physics
@@ -622,6 +623,10 @@ __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
 
   // -------------------------------------------------------------- prologue
   if (!a.prologue) return;
+  if (a.ystage) {  // the stage state itself, for the closure's features
+#pragma unroll
+    for (int m = 0; m < 8; ++m) a.ystage[ro + t + m * Q] = y[m];
+  }
 
   // <theta f> partials of the prologue state (ring_forcing.py:153-155 by
   // Parseval over the ring modes; weight 2 for each +/-k pair).
@@ -827,6 +832,7 @@ __global__ void __launch_bounds__(RKShape<N>::NT) rk_kernel(RKArgs a) {
   constexpr int M = S::M, Q = S::Q, WX = S::WX, NT = S::NT, PADM = S::PADM, RG = S::REGION;
   constexpr int NF = RKFields<JOB>::value;
   constexpr int G = S::FP < NF ? S::FP : NF;  // fields per lockstep group
+  constexpr bool THETA = (JOB == RK_THETA || JOB == RK_THETA_CL);
   // Twiddles and pre/post-twist factors from one LDS octant table of
   // W_N where it is small (N <= 2048); W_M = W_N^2.
   constexpr bool TWL = N <= 2048;
@@ -878,10 +884,18 @@ __global__ void __launch_bounds__(RKShape<N>::NT) rk_kernel(RKArgs a) {
     const cd* src[G];
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-      const int q = (JOB == RK_THETA) ? ((g & 1) ? 2 + (g >> 1) : (g >> 1)) : g;
+      const int q = THETA ? ((g & 1) ? 2 + (g >> 1) : (g >> 1)) : g;
       src[g] = a.F[q] + fo;
     }
     rk_issue<N, G>(src, x0, tid, tmp);
+  }
+  // closure flux of this x-line as packed real pairs (y = 2k, 2k + 1)
+  typedef float vf2 __attribute__((vector_size(8)));
+  vf2 tz[JOB == RK_THETA_CL ? 8 : 1];
+  if (JOB == RK_THETA_CL) {
+    const vf2* tp = reinterpret_cast<const vf2*>(a.tau + (size_t)x * N);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) tz[m] = tp[t + m * Q];
   }
   if (JOB == RK_SCALAR) {
     const cd* up = reinterpret_cast<const cd*>(a.uphys + (size_t)x * N);
@@ -924,7 +938,7 @@ __global__ void __launch_bounds__(RKShape<N>::NT) rk_kernel(RKArgs a) {
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         const int s = g0 + G + g;
-        const int q = (JOB == RK_THETA) ? ((s & 1) ? 2 + (s >> 1) : (s >> 1)) : s;
+        const int q = THETA ? ((s & 1) ? 2 + (s >> 1) : (s >> 1)) : s;
         src[g] = a.F[q] + fo;
       }
       rk_issue<N, G>(src, x0, tid, tmp);
@@ -937,7 +951,7 @@ __global__ void __launch_bounds__(RKShape<N>::NT) rk_kernel(RKArgs a) {
         cd* dst = reinterpret_cast<cd*>(dstp + fo + (size_t)x * N);
 #pragma unroll
         for (int m = 0; m < 8; ++m) dst[t + m * Q] = w[g][m];
-      } else if (JOB == RK_THETA) {
+      } else if (THETA) {
         if ((s & 1) == 0) {
 #pragma unroll
           for (int m = 0; m < 8; ++m) hold[m] = w[g][m];
@@ -969,6 +983,11 @@ __global__ void __launch_bounds__(RKShape<N>::NT) rk_kernel(RKArgs a) {
     }
   }
   if (JOB == RK_C2R || JOB == RK_VEL) return;
+  if (JOB == RK_THETA_CL) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+      jz[m] = cmk(jz[m].x + (double)tz[m][0], jz[m].y + (double)tz[m][1]);
+  }
 #pragma unroll
   for (int m = 0; m < 8; ++m) v[m] = jz[m];
   __syncthreads();
@@ -1395,6 +1414,7 @@ hipError_t rk_by_job(int job, const RKArgs& a, int ny, hipStream_t s, const Laun
     case RK_VEL: return launch_rk_n<N, RK_VEL>(a, ny, s, ev);
     case RK_R2C: return launch_rk_n<N, RK_R2C>(a, ny, s, ev);
     case RK_C2R: return launch_rk_n<N, RK_C2R>(a, ny, s, ev);
+    case RK_THETA_CL: return launch_rk_n<N, RK_THETA_CL>(a, ny, s, ev);
   }
   return hipErrorInvalidValue;
 }

@@ -1,6 +1,7 @@
 """CPU tests of the host side: the library loads and exports every symbol
-include/gsqg.h declares (no compute calls), and the host tables (wavenumbers,
-masks, flag packing, ring list, keys, initial conditions) match the oracle."""
+include/gsqg.h and include/gsqg_closure.h declare (no compute calls), and the
+host tables (wavenumbers, masks, flag packing, ring list, keys, initial
+conditions) match the oracle."""
 
 import os
 import re
@@ -16,8 +17,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def header_symbols():
-    txt = open(os.path.join(ROOT, "include", "gsqg.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(gsqg_\w+)\s*\(", txt, re.M)))
+    syms = set()
+    for h in ("gsqg.h", "gsqg_closure.h"):
+        txt = open(os.path.join(ROOT, "include", h)).read()
+        syms |= set(re.findall(r"^\s*(?:int|const char\*)\s+(gsqg_\w+)\s*\(", txt, re.M))
+    return sorted(syms)
 
 
 def test_library_exports_every_header_symbol():
