@@ -19,7 +19,15 @@ set (each its own JSON line, same keys):
          mu=0.1, nu_p=1e-8*(512/N)^8, p=8, 4 Gaussian scalars, dt=1e-4;
   batch  96 members, 32 each at N=128, 256, 512, alpha cycling
          {0, 0.5, 1, 1.5, 2}, seeded k^-5/3 theta0, nu_p=1e-4, p=8, dt=1e-3,
-         against looping the single-realization solver over the members.
+         against looping the single-realization solver over the members;
+  closure  learned subgrid closure: forced DNS at N=4096 (alpha=1, ring
+         kf=40 dk=2 eps=0.1, mu=0.1, nu_p=1e-8*(512/N)^8, p=8, dt=1e-4),
+         snapshots sharp-filtered to N=512 with the filtered-minus-resolved
+         Jacobian flux as target, the CNN trained on the GPU (batch 32, Adam
+         lr=1e-3), then the closed N=512 run against the filtered DNS and the
+         unclosed N=512 run: shell-spectrum error for k <= 170 and wall-clock
+         per simulated time unit.  --nf/--nc/--snaps/--train-steps/--t-eval
+         shrink it for quick checks.
 """
 
 from __future__ import annotations
@@ -82,7 +90,17 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--copy-only", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates PMC byte counters)")
-    ap.add_argument("--config", default="forced", choices=["forced", "plain", "large", "batch"])
+    ap.add_argument("--config", default="forced",
+                    choices=["forced", "plain", "large", "batch", "closure"])
+    # closure workload sizes
+    ap.add_argument("--nf", type=int, default=4096)
+    ap.add_argument("--nc", type=int, default=512)
+    ap.add_argument("--snaps", type=int, default=2000)
+    ap.add_argument("--snap-every", type=int, default=5)
+    ap.add_argument("--train-steps", type=int, default=1500)
+    ap.add_argument("--t-spin", type=float, default=0.5)
+    ap.add_argument("--t-eval", type=float, default=10.0)
+    ap.add_argument("--dt-coarse", type=float, default=5e-4)
     args = ap.parse_args()
     if args.copy_only:
         import torch
@@ -101,6 +119,8 @@ def main():
         sys.exit(2)
     if args.config == "batch":
         return bench_batch(args)
+    if args.config == "closure":
+        return bench_closure(args)
     if args.config in ("plain", "large"):
         return bench_single(args)
 
@@ -322,6 +342,165 @@ def bench_batch(args):
                           "steps": loop_steps, "speedup": value / loop_value},
         "cpu_baseline": None,
         "note": "cpu_baseline is reported for the default configuration only"}))
+
+
+def bench_closure(args):
+    """DNS -> filtered snapshots -> training -> closed vs filtered DNS."""
+    import numpy as np
+    import torch
+
+    import pygsquig_amd as pg
+    from pygsquig_amd import closure as C
+    from pygsquig_amd import ics
+
+    dev = torch.device("cuda:0")
+    L, alpha = 2 * np.pi, 1.0
+    Nf, Nc = args.nf, args.nc
+    dt_f, dt_c = 1e-4, args.dt_coarse
+    nu = 1e-8 * (512 / Nf) ** 8
+    kf, dk, eps, mu = 40.0, 2.0, 0.1, 0.1
+    t_start = time.perf_counter()
+
+    def log(msg):
+        print(f"[closure {time.perf_counter() - t_start:7.1f}s] {msg}", flush=True)
+
+    def sync():
+        torch.cuda.synchronize()
+
+    gf, gc = pg.make_grid(Nf, L, dev), pg.make_grid(Nc, L, dev)
+    forcing_f = pg.RingForcing(kf=kf, dk=dk, epsilon=eps, seed=42)
+    damping = pg.CombinedDamping(mu=mu, kf=kf, nu_p=nu, p=8)
+    dns = pg.gSQGSolver(gf, alpha, 0.0, 8, scheme="if")
+    st = dns.initialize(torch.from_numpy(ics.kolmogorov_53(Nf, seed=2, alpha=alpha)).to(dev))
+    dns_sec = 0.0
+
+    def dns_run(state, n):
+        nonlocal dns_sec
+        sync()
+        t0 = time.perf_counter()
+        state = dns.multistep(state, n, dt_f, forcing=forcing_f, damping=damping)
+        sync()
+        dns_sec += time.perf_counter() - t0
+        return state
+
+    n_spin = int(round(args.t_spin / dt_f))
+    chunk = 2000
+    for i in range(0, n_spin, chunk):
+        st = dns_run(st, min(chunk, n_spin - i))
+        log(f"DNS N={Nf} spin-up step {min(i + chunk, n_spin)}/{n_spin}")
+    # filtered snapshots
+    sampler = C.SubgridSampler(gf, gc, alpha)
+    store = C.SnapshotStore(args.snaps, Nc, dev)
+    for k in range(args.snaps):
+        st = dns_run(st, args.snap_every)
+        _, th, u, v, tau = sampler.sample(st["theta_hat"])
+        store.add(th, u, v, tau)
+        if (k + 1) % 250 == 0 or k + 1 == args.snaps:
+            log(f"snapshot {k + 1}/{args.snaps}")
+    in_scale, tgt_scale = store.scales()
+    rms_j = float(torch.sqrt(torch.mean(store.tau[: store.count] ** 2)))
+    # training
+    net = C.ClosureNet(Nc, Nc, max_batch=32, device=dev, seed=0)
+    sync()
+    t0 = time.perf_counter()
+    hist = C.train(net, store, args.train_steps, batch=32, lr=1e-3, in_scale=in_scale,
+                   tgt_scale=tgt_scale, seed=0, log_every=100,
+                   log=lambda m: log(m))
+    sync()
+    train_sec = time.perf_counter() - t0
+    flop_fwd = 2.0 * Nc * Nc * (27 * 64 + 4 * 576 * 64 + 576)
+    train_tflops = 3 * flop_fwd * 32 * args.train_steps / train_sec / 1e12
+    # inference throughput of one coarse field
+    th1, u1, v1 = store.theta[:1], store.u[:1], store.v[:1]
+    for _ in range(3):
+        net.forward(th1, u1, v1, in_scale, 1.0)
+    sync()
+    t0 = time.perf_counter()
+    n_inf = 50
+    for _ in range(n_inf):
+        net.forward(th1, u1, v1, in_scale, 1.0)
+    sync()
+    inf_ms = (time.perf_counter() - t0) / n_inf * 1e3
+    log(f"trained: loss {hist[0][1]:.3e} -> {hist[-1][1]:.3e}; {train_tflops:.1f} TFLOP/s "
+        f"training, forward {inf_ms:.3f} ms")
+    # evaluation over t_eval: filtered DNS vs closed vs unclosed coarse runs
+    t_eval = args.t_eval
+    sample_dt = max(t_eval / 100, 0.01)
+    n_samp = int(round(t_eval / sample_dt))
+    nf_per = int(round(sample_dt / dt_f))
+    nc_per = int(round(sample_dt / dt_c))
+    thb0 = C.spectral_filter(st["theta_hat"], gc)
+    forcing_c = pg.RingForcing(kf=kf, dk=dk, epsilon=eps, seed=43)
+    closed = C.ClosedSolver(gc, alpha, 0.0, 8, net=net, in_scale=in_scale,
+                            out_scale=1.0 / tgt_scale, scheme="if")
+    plain = pg.gSQGSolver(gc, alpha, 0.0, 8, scheme="if")
+    kmax = int(Nc // 3)
+
+    def shell(th_c):
+        return pg.compute_shell_spectrum(th_c, gc, alpha)[: kmax + 1]
+
+    def run_coarse(solver):
+        s = solver.initialize()
+        s = {"theta_hat": thb0.clone(), "time": s["time"], "step": s["step"]}
+        acc = np.zeros(kmax + 1)
+        sec = 0.0
+        for i in range(n_samp):
+            sync()
+            t0 = time.perf_counter()
+            s = solver.multistep(s, nc_per, dt_c, forcing=forcing_c, damping=damping)
+            sync()
+            sec += time.perf_counter() - t0
+            acc += shell(s["theta_hat"])
+        return acc / n_samp, sec, bool(torch.isfinite(torch.view_as_real(s["theta_hat"])).all())
+
+    E_cl, sec_cl, fin_cl = run_coarse(closed)
+    log(f"closed N={Nc}: {sec_cl:.2f} s for {t_eval} time units")
+    E_pl, sec_pl, fin_pl = run_coarse(plain)
+    log(f"unclosed N={Nc}: {sec_pl:.2f} s")
+    dns_sec = 0.0
+    E_dns = np.zeros(kmax + 1)
+    for i in range(n_samp):
+        st = dns_run(st, nf_per)
+        E_dns += shell(C.spectral_filter(st["theta_hat"], gc))
+        if (i + 1) % 10 == 0:
+            log(f"DNS evaluation {i + 1}/{n_samp} samples")
+    E_dns /= n_samp
+    dns_eval_sec = dns_sec
+    ks = np.arange(1, kmax + 1)
+    err_cl = np.abs(E_cl[ks] - E_dns[ks]) / E_dns[ks]
+    err_pl = np.abs(E_pl[ks] - E_dns[ks]) / E_dns[ks]
+    speedup = dns_eval_sec / sec_cl
+    out = {
+        "metric": "closure speedup at matched E(k) error", "value": speedup,
+        "unit": "x (filtered-DNS wall-clock / closed-run wall-clock, same simulated time)",
+        "n_gpus": 1, "steps": n_samp * nc_per, "warmup": 0,
+        "ms_per_step": sec_cl / (n_samp * nc_per) * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 solver, bf16/fp32 network",
+        "data": "synthetic (the solver's own forced DNS)",
+        "config": {"workload": f"learned closure: DNS N={Nf} -> N={Nc}", "Nf": Nf, "Nc": Nc,
+                   "alpha": alpha, "dt_dns": dt_f, "dt_coarse": dt_c, "nu_p": nu,
+                   "snapshots": store.count, "snap_every": args.snap_every,
+                   "train_steps": args.train_steps, "t_spin": args.t_spin, "t_eval": t_eval,
+                   "finite": fin_cl and fin_pl},
+        "spectrum_error": {
+            "k_max": kmax,
+            "closed_mean_rel": float(err_cl.mean()), "closed_max_rel": float(err_cl.max()),
+            "unclosed_mean_rel": float(err_pl.mean()), "unclosed_max_rel": float(err_pl.max()),
+            "closed_within_5pct_frac": float(np.mean(err_cl <= 0.05)),
+            "unclosed_within_5pct_frac": float(np.mean(err_pl <= 0.05))},
+        "wall_clock_s": {"dns_eval": dns_eval_sec, "closed": sec_cl, "unclosed": sec_pl,
+                         "training": train_sec},
+        "training": {"loss_first": hist[0][1], "loss_last": hist[-1][1],
+                     "tflops": train_tflops, "mfma_frac": train_tflops / 2500.0,
+                     "forward_ms": inf_ms,
+                     "forward_tflops": flop_fwd / (inf_ms * 1e-3) / 1e12,
+                     "tau_rms": rms_j},
+        "spectra": {"k": ks.tolist(), "dns": E_dns[ks].tolist(), "closed": E_cl[ks].tolist(),
+                    "unclosed": E_pl[ks].tolist()},
+        "cpu_baseline": None,
+        "note": "cpu_baseline is reported for the default configuration only",
+    }
+    print(json.dumps(out))
 
 
 if __name__ == "__main__":
