@@ -22,22 +22,38 @@ namespace cl {
 // ------------------------------------------------------------ helpers
 __device__ __forceinline__ float bf2f(uint32_t h) { return __uint_as_float(h << 16); }
 
-// float -> bfloat16, round to nearest, ties to even (quiet NaN kept).
-__device__ __forceinline__ uint32_t f2bf(float f) {
-  const uint32_t u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (u >> 16) | 0x40u;
-  return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
-}
+// float -> bfloat16, round to nearest, ties to even.
+__device__ __forceinline__ uint32_t f2bf(float f) { return bf16x2_rne(f, 0.0f) & 0xffffu; }
 
 __device__ __forceinline__ float rbf(float f) { return bf2f(f2bf(f)); }
 
-// GELU (erf form) and its derivative, float32.
-__device__ __forceinline__ float gelu(float z) {
-  return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f));
+// Standard normal CDF Phi(z) = (1 + erf(z / sqrt 2)) / 2 and density
+// phi(z), branch-free: erfc from Abramowitz & Stegun 7.1.26 (rational
+// approximation, absolute error <= 1.5e-7, relative error of the lower tail
+// below 5e-4 for z > -4, where |gelu(z)| > 5e-4; both below the resolution
+// of the bf16 values it produces), exp(-x^2) from the hardware exp2.
+struct NormalCdfPdf {
+  float cdf, pdf;
+};
+__device__ __forceinline__ NormalCdfPdf normal_cdf_pdf(float z) {
+  const float x = fabsf(z) * 0.70710678118654752f;  // |z| / sqrt 2
+  const float t = __frcp_rn(1.0f + 0.3275911f * x);
+  const float poly =
+      t * (0.254829592f +
+           t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
+  const float e = exp2f(-x * x * 1.4426950408889634f);  // exp(-x^2)
+  const float tail = 0.5f * poly * e;                    // Phi(-|z|) = erfc(|z| / sqrt 2) / 2
+  NormalCdfPdf r;
+  r.cdf = z < 0.0f ? tail : 1.0f - tail;  // the lower tail without cancellation
+  r.pdf = 0.39894228040143268f * e;       // exp(-z^2 / 2) / sqrt(2 pi)
+  return r;
 }
+
+// GELU (erf form) and its derivative, float32.
+__device__ __forceinline__ float gelu(float z) { return z * normal_cdf_pdf(z).cdf; }
 __device__ __forceinline__ float gelu_d(float z) {
-  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) +
-         z * expf(-0.5f * z * z) * 0.39894228040143268f;
+  const NormalCdfPdf n = normal_cdf_pdf(z);
+  return n.cdf + z * n.pdf;
 }
 
 __device__ __forceinline__ float unpack(const u32x4& v, int e) {
@@ -47,7 +63,7 @@ __device__ __forceinline__ float unpack(const u32x4& v, int e) {
 __device__ __forceinline__ u32x4 pack8(const float (&f)[8]) {
   u32x4 r;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) r[k] = f2bf(f[2 * k]) | (f2bf(f[2 * k + 1]) << 16);
+  for (int k = 0; k < 4; ++k) r[k] = bf16x2_rne(f[2 * k], f[2 * k + 1]);
   return r;
 }
 
@@ -137,26 +153,33 @@ __global__ void __launch_bounds__(256) conv_in_fwd(Feat f, const float* __restri
 //   z = bf16(out + bias), a = bf16(gelu(z)).
 // MODE 1 (backward data): in = dZ of the layer, Wt = Wb (taps reversed, rows
 //   = input channels); out = bf16(out * gelu'(zprev)) is dZ of the layer below.
+// LDS holds the 6 x 66 halo tile and the three taps of one kernel row at a
+// time (75 KB: two blocks share a CU, so one block's epilogue overlaps the
+// other's MFMAs).  Rows of 64 channels are 128 bytes with their eight
+// 16-byte chunks XOR-swizzled by (row & 7), so the MFMA operand reads of
+// eight consecutive rows hit eight different bank groups.
+constexpr int MID_IN = HR * HC * C;  // bf16 elements of the staged tile
+constexpr int MID_W = 3 * C * C;     // one kernel row (3 taps)
+
+__device__ __forceinline__ int swz(int row, int chunk) { return row * C + ((chunk ^ (row & 7)) << 3); }
+
 template <int MODE>
-__global__ void __launch_bounds__(256) conv_mid(const uint16_t* __restrict__ in,
-                                                const uint16_t* __restrict__ Wt,
-                                                const float* __restrict__ bias,
-                                                const uint16_t* __restrict__ zprev, uint16_t* zout,
-                                                uint16_t* aout, int H, int W) {
-  __shared__ __attribute__((aligned(16))) uint16_t lds[IN_E + W_E];
-  uint16_t* li = lds;
-  uint16_t* lw = lds + IN_E;
+__global__ void __launch_bounds__(256, 2) conv_mid(const uint16_t* __restrict__ in,
+                                                   const uint16_t* __restrict__ Wt,
+                                                   const float* __restrict__ bias,
+                                                   const uint16_t* __restrict__ zprev,
+                                                   uint16_t* zout, uint16_t* aout, int H, int W) {
+  __shared__ __attribute__((aligned(16))) uint16_t li[MID_IN];
+  __shared__ __attribute__((aligned(16))) uint16_t lw[MID_W];
   const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
   const int j0 = blockIdx.x * TC, i0 = blockIdx.y * TR, b = blockIdx.z;
   const size_t ib = (size_t)b * H;
-  stage_tile(li, in, ib, i0, j0, H, W, tid);
-  for (int e = tid; e < 9 * C * 8; e += 256) {
-    const int row = e >> 3, h = e & 7;
-    *reinterpret_cast<u32x4*>(lw + row * PS + 8 * h) =
-        *reinterpret_cast<const u32x4*>(Wt + row * C + 8 * h);
+  for (int e = tid; e < HR * HC * 8; e += 256) {
+    const int q = e >> 3, h = e & 7, r = q / HC, cc = q - r * HC;
+    const int gi = wrap(i0 - 1 + r, H), gj = wrap(j0 - 1 + cc, W);
+    *reinterpret_cast<u32x4*>(li + swz(q, h)) =
+        *reinterpret_cast<const u32x4*>(in + ((ib + gi) * W + gj) * C + 8 * h);
   }
-  __syncthreads();
-
   f32x16 acc[2][2];
 #pragma unroll
   for (int h = 0; h < 2; ++h)
@@ -164,66 +187,74 @@ __global__ void __launch_bounds__(256) conv_mid(const uint16_t* __restrict__ in,
     for (int g = 0; g < 2; ++g)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[h][g][r] = 0.0f;
-  const int lr = l & 31, lk = (l >> 5) * 8;
-  for (int t = 0; t < 9; ++t) {
-    const int dy = t / 3 - 1, dx = t % 3 - 1;
-    const uint16_t* wa = lw + (t * C + lr) * PS + lk;
-    const uint16_t* pb = li + ((wave + 1 + dy) * HC + 1 + dx + lr) * PS + lk;
+  const int lr = l & 31, lh = l >> 5;
+  for (int ky = 0; ky < 3; ++ky) {
+    if (ky > 0) __syncthreads();  // every wave is done with the previous kernel row
+    for (int e = tid; e < 3 * C * 8; e += 256) {
+      const int row = e >> 3, h = e & 7;
+      *reinterpret_cast<u32x4*>(lw + swz(row, h)) =
+          *reinterpret_cast<const u32x4*>(Wt + (size_t)(3 * ky * C + row) * C + 8 * h);
+    }
+    __syncthreads();
 #pragma unroll
-    for (int kc = 0; kc < 4; ++kc) {
-      const u32x4 A0 = *reinterpret_cast<const u32x4*>(wa + 16 * kc);
-      const u32x4 A1 = *reinterpret_cast<const u32x4*>(wa + 32 * PS + 16 * kc);
-      const u32x4 B0 = *reinterpret_cast<const u32x4*>(pb + 16 * kc);
-      const u32x4 B1 = *reinterpret_cast<const u32x4*>(pb + 32 * PS + 16 * kc);
-      acc[0][0] = mfma_32x32x16(A0, B0, acc[0][0]);
-      acc[0][1] = mfma_32x32x16(A0, B1, acc[0][1]);
-      acc[1][0] = mfma_32x32x16(A1, B0, acc[1][0]);
-      acc[1][1] = mfma_32x32x16(A1, B1, acc[1][1]);
+    for (int kx = 0; kx < 3; ++kx) {
+      const int wr0 = kx * C + lr, wr1 = wr0 + 32;
+      const int q0 = (wave + ky) * HC + kx + lr, q1 = q0 + 32;
+#pragma unroll
+      for (int kc = 0; kc < 4; ++kc) {
+        const int ch = 2 * kc + lh;
+        const u32x4 A0 = *reinterpret_cast<const u32x4*>(lw + swz(wr0, ch));
+        const u32x4 A1 = *reinterpret_cast<const u32x4*>(lw + swz(wr1, ch));
+        const u32x4 B0 = *reinterpret_cast<const u32x4*>(li + swz(q0, ch));
+        const u32x4 B1 = *reinterpret_cast<const u32x4*>(li + swz(q1, ch));
+        acc[0][0] = mfma_32x32x16(A0, B0, acc[0][0]);
+        acc[0][1] = mfma_32x32x16(A0, B1, acc[0][1]);
+        acc[1][0] = mfma_32x32x16(A1, B0, acc[1][0]);
+        acc[1][1] = mfma_32x32x16(A1, B1, acc[1][1]);
+      }
     }
   }
-  __syncthreads();  // every wave is done with the staged tile
-
-  // accumulators -> float staging [pixel][channel] -> coalesced stores
-  float* os = reinterpret_cast<float*>(lds);
+  // epilogue straight from the accumulators: lane (lr, lh) holds channels
+  // co0 .. co0 + 3 (co0 = 32 h + 8 q + 4 lh) of pixel 32 g + lr of row `wave`
+  const size_t prow = (ib + i0 + wave) * W + j0;
 #pragma unroll
   for (int h = 0; h < 2; ++h)
 #pragma unroll
-    for (int g = 0; g < 2; ++g)
+    for (int q = 0; q < 4; ++q) {
+      const int co0 = 32 * h + 8 * q + 4 * lh;
+      float bb[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      if (MODE == 0) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int co0 = 32 * h + 8 * q + 4 * (l >> 5);
-        const int px = wave * TC + 32 * g + lr;
+        for (int e = 0; e < 4; ++e) bb[e] = bias[co0 + e];
+      }
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          float v = acc[h][g][4 * q + e];
-          if (MODE == 0) v += bias[co0 + e];
-          os[px * OS + co0 + e] = v;
+      for (int g = 0; g < 2; ++g) {
+        const size_t o = (prow + 32 * g + lr) * C + co0;
+        float v[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[e] = acc[h][g][4 * q + e];
+        uint32_t w0, w1;
+        if (MODE == 0) {
+          const uint32_t z0 = bf16x2_rne(v[0] + bb[0], v[1] + bb[1]);
+          const uint32_t z1 = bf16x2_rne(v[2] + bb[2], v[3] + bb[3]);
+          if (zout) {
+            uint32_t* zp = reinterpret_cast<uint32_t*>(zout + o);
+            zp[0] = z0;
+            zp[1] = z1;
+          }
+          w0 = bf16x2_rne(gelu(bf2f(z0 & 0xffffu)), gelu(bf2f(z0 >> 16)));
+          w1 = bf16x2_rne(gelu(bf2f(z1 & 0xffffu)), gelu(bf2f(z1 >> 16)));
+        } else {
+          const uint32_t* zq = reinterpret_cast<const uint32_t*>(zprev + o);
+          const uint32_t p0 = zq[0], p1 = zq[1];
+          w0 = bf16x2_rne(v[0] * gelu_d(bf2f(p0 & 0xffffu)), v[1] * gelu_d(bf2f(p0 >> 16)));
+          w1 = bf16x2_rne(v[2] * gelu_d(bf2f(p1 & 0xffffu)), v[3] * gelu_d(bf2f(p1 >> 16)));
         }
+        uint32_t* ap = reinterpret_cast<uint32_t*>(aout + o);
+        ap[0] = w0;
+        ap[1] = w1;
       }
-  __syncthreads();
-  for (int e = tid; e < TR * TC * 8; e += 256) {
-    const int px = e >> 3, h = e & 7;
-    const size_t pix = (ib + i0 + (px >> 6)) * W + j0 + (px & 63);
-    float v[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = os[px * OS + 8 * h + k];
-    if (MODE == 0) {
-      float zf[8], af[8];
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        zf[k] = rbf(v[k]);
-        af[k] = gelu(zf[k]);
-      }
-      if (zout) *reinterpret_cast<u32x4*>(zout + pix * C + 8 * h) = pack8(zf);
-      *reinterpret_cast<u32x4*>(aout + pix * C + 8 * h) = pack8(af);
-    } else {
-      const u32x4 zp = *reinterpret_cast<const u32x4*>(zprev + pix * C + 8 * h);
-      float d[8];
-#pragma unroll
-      for (int k = 0; k < 8; ++k) d[k] = v[k] * gelu_d(unpack(zp, k));
-      *reinterpret_cast<u32x4*>(aout + pix * C + 8 * h) = pack8(d);
     }
-  }
 }
 
 // --------------------------------------------------- layer 5 forward
@@ -282,7 +313,9 @@ __global__ void __launch_bounds__(256) loss_kernel(const float* __restrict__ tau
 // ------------------------------------------------- layer 5 backward
 // dz4 = bf16(gelu'(z4) * sum_t W5r[t] dtau[p - off_t]); partial
 // dW5[t][ci] = sum_p a4[p + off_t][ci] dtau[p] and db5 = sum_p dtau[p] per
-// block (persistent over 4 x 64 tiles).
+// block (persistent over 4 x 64 tiles).  The weight-gradient sums run as
+// 144 tasks (tap t, channel group of 8, half tile), each reading 8 channels
+// per pixel with one 16-byte LDS read.
 __global__ void __launch_bounds__(256) conv_out_bwd(const uint16_t* __restrict__ a,
                                                     const uint16_t* __restrict__ z,
                                                     const float* __restrict__ dtau,
@@ -291,10 +324,14 @@ __global__ void __launch_bounds__(256) conv_out_bwd(const uint16_t* __restrict__
   __shared__ __attribute__((aligned(16))) uint16_t li[IN_E];
   __shared__ float ds[HR][HC];
   __shared__ float ws[NW5];
+  __shared__ float half1[72][8];
   const int tid = threadIdx.x;
   for (int e = tid; e < NW5; e += 256) ws[e] = W5r[e];
   const int tx_n = W / TC, ty_n = H / TR, ntiles = B * tx_n * ty_n;
-  float gacc[3] = {0.0f, 0.0f, 0.0f};
+  // weight-gradient task of this thread (tid < 144), db5 (tid == 144)
+  const int cg = tid & 7, th = tid >> 3, tt = th >> 1, hf = th & 1;
+  float g8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  float gb = 0.0f;
   const int r = tid >> 6, c = tid & 63;
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int tx = tile % tx_n, rest = tile / tx_n, ty = rest % ty_n, b = rest / ty_n;
@@ -324,28 +361,38 @@ __global__ void __launch_bounds__(256) conv_out_bwd(const uint16_t* __restrict__
       }
       *reinterpret_cast<u32x4*>(dz + pix * C + 8 * h) = pack8(d);
     }
+    if (tid < 144) {
+      const int dy = tt / 3 - 1, dx = tt % 3 - 1;
+      for (int rr = 2 * hf; rr < 2 * hf + 2; ++rr) {
+#pragma unroll 4
+        for (int cc = 0; cc < TC; ++cc) {
+          const u32x4 v = *reinterpret_cast<const u32x4*>(
+              li + ((rr + 1 + dy) * HC + cc + 1 + dx) * PS + 8 * cg);
+          const float dt = ds[rr + 1][cc + 1];
 #pragma unroll
-    for (int n = 0; n < 3; ++n) {
-      const int o = tid + 256 * n;
-      if (o < NW5) {
-        const int t = o >> 6, ci = o & 63, dy = t / 3 - 1, dx = t % 3 - 1;
-        float s = 0.0f;
-        for (int rr = 0; rr < TR; ++rr)
-          for (int cc = 0; cc < TC; ++cc)
-            s += bf2f(li[((rr + 1 + dy) * HC + cc + 1 + dx) * PS + ci]) * ds[rr + 1][cc + 1];
-        gacc[n] += s;
-      } else if (o == NW5) {
-        float s = 0.0f;
-        for (int rr = 0; rr < TR; ++rr)
-          for (int cc = 0; cc < TC; ++cc) s += ds[rr + 1][cc + 1];
-        gacc[n] += s;
+          for (int k = 0; k < 8; ++k) g8[k] += unpack(v, k) * dt;
+        }
       }
+    } else if (tid == 144) {
+      float sb = 0.0f;
+      for (int rr = 0; rr < TR; ++rr)
+        for (int cc = 0; cc < TC; ++cc) sb += ds[rr + 1][cc + 1];
+      gb += sb;
     }
   }
+  // combine the two half-tile tasks of each (tap, channel group)
+  __syncthreads();
+  if (tid < 144 && hf == 1) {
 #pragma unroll
-  for (int n = 0; n < 3; ++n) {
-    const int o = tid + 256 * n;
-    if (o < PART_OUT) part[(size_t)blockIdx.x * PART_OUT + o] = gacc[n];
+    for (int k = 0; k < 8; ++k) half1[tt * 8 + cg][k] = g8[k];
+  }
+  __syncthreads();
+  float* pp = part + (size_t)blockIdx.x * PART_OUT;
+  if (tid < 144 && hf == 0) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) pp[tt * C + 8 * cg + k] = g8[k] + half1[tt * 8 + cg][k];
+  } else if (tid == 144) {
+    pp[NW5] = gb;
   }
 }
 
@@ -449,56 +496,92 @@ __global__ void __launch_bounds__(256) conv_wgrad(const uint16_t* __restrict__ a
 }
 
 // ------------------------------------------------ layer 0 weight gradient
-// dW0[t][ci][co] = sum_p x[p + off_t][ci] dz0[p][co], db0 = sum_p dz0[p][co];
-// thread (co = tid & 63, group g = tid >> 6) owns the (t, ci) pairs
-// k = g, g + 4, ... < 27.  Persistent over 4 x 64 tiles.
+// dW0[t][ci][co] = sum_p x[p + off_t][ci] dz0[p][co] and db0 = sum_p dz0[p][co]
+// as one MFMA GEMM: rows m = 3 t + ci (m < 27) of the bf16 inputs shifted by
+// tap t, row 27 all ones (its products sum dz0: db0), rows 28..31 zero;
+// columns co; reduction over pixels.  Persistent over 4 x 64 tiles; wave w
+// takes tile row w.  LDS: the inputs as three column-shifted copies
+// [dx][ci][row][64 px] and dz0 channel-major.
+constexpr int IW_DS = TR * 64 + 8;  // bf16 per channel of the transposed dz0 tile
+
 __global__ void __launch_bounds__(256) conv_in_wgrad(Feat f, const uint16_t* __restrict__ dz,
                                                      float* part, int B, int H, int W) {
-  __shared__ float xs[3][HR][HC];
-  __shared__ __attribute__((aligned(16))) uint16_t ds[TR * TC * C];
-  const int tid = threadIdx.x, co = tid & 63, g = tid >> 6;
+  __shared__ __attribute__((aligned(16))) uint16_t xsb[3][3][HR][64];
+  __shared__ __attribute__((aligned(16))) uint16_t dzt[C * IW_DS];
+  const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
+  const int lr = l & 31, lk = (l >> 5) * 8;
   const int tx_n = W / TC, ty_n = H / TR, ntiles = B * tx_n * ty_n;
-  float acc[7] = {0, 0, 0, 0, 0, 0, 0};
-  float accb = 0.0f;
+  // A-operand row of this lane
+  const int m = lr;
+  const int mt = m < 27 ? m / 3 : 0, mci = m < 27 ? m - 3 * (m / 3) : 0;
+  const int mdy = mt / 3 - 1, mdx = mt % 3 - 1;
+  f32x16 acc[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[h][q] = 0.0f;
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int tx = tile % tx_n, rest = tile / tx_n, ty = rest % ty_n, b = rest / ty_n;
     const int i0 = ty * TR, j0 = tx * TC;
     const long long img = f.sel ? f.sel[b] : b;
     __syncthreads();
-    for (int e = tid; e < 3 * HR * HC; e += 256) {
-      const int c = e / (HR * HC), q = e - c * (HR * HC), r = q / HC, cc = q - r * HC;
-      const int gi = wrap(i0 - 1 + r, H), gj = wrap(j0 - 1 + cc, W);
-      const double v = f.plane[c][img * f.img_stride + (long long)gi * W + gj] * f.in_scale[c];
-      xs[c][r][cc] = rbf((float)v);
+    for (int e = tid; e < 3 * 3 * HR * 64; e += 256) {
+      const int px = e & 63, q = e >> 6, rr = q % HR, q2 = q / HR, ci = q2 % 3, sx = q2 / 3;
+      const int gi = wrap(i0 - 1 + rr, H), gj = wrap(j0 + px + sx - 1, W);
+      const double v = f.plane[ci][img * f.img_stride + (long long)gi * W + gj] * f.in_scale[ci];
+      xsb[sx][ci][rr][px] = (uint16_t)f2bf((float)v);
     }
-    for (int e = tid; e < TR * TC * 8; e += 256) {
-      const int px = e >> 3, h = e & 7;
-      const size_t pix = ((size_t)b * H + i0 + (px >> 6)) * W + j0 + (px & 63);
-      *reinterpret_cast<u32x4*>(ds + px * C + 8 * h) =
-          *reinterpret_cast<const u32x4*>(dz + pix * C + 8 * h);
+    {
+      const int mm = tid & 7, cgp = (tid >> 3) & 7, rr = tid >> 6;
+      u32x4 vin[8], vout[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        vin[e] = *reinterpret_cast<const u32x4*>(
+            dz + (((size_t)b * H + i0 + rr) * W + j0 + 8 * mm + e) * C + 8 * cgp);
+      transpose8(vin, vout);
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        *reinterpret_cast<u32x4*>(dzt + (8 * cgp + e) * IW_DS + rr * 64 + 8 * mm) = vout[e];
     }
     __syncthreads();
-    for (int px = 0; px < TR * TC; ++px) {
-      const int r = px >> 6, c = px & 63;
-      const float d = bf2f(ds[px * C + co]);
 #pragma unroll
-      for (int n = 0; n < 7; ++n) {
-        const int k = g + 4 * n;
-        if (k < 27) {
-          const int t = k / 3, ci = k - 3 * t;
-          acc[n] += xs[ci][r + 1 + t / 3 - 1][c + 1 + t % 3 - 1] * d;
-        }
+    for (int ks = 0; ks < 4; ++ks) {
+      const int px0 = 16 * ks + lk;
+      u32x4 Av;
+      if (m < 27) {
+        Av = *reinterpret_cast<const u32x4*>(&xsb[mdx + 1][mci][wave + 1 + mdy][px0]);
+      } else {
+        const unsigned one = (m == 27) ? 0x3F803F80u : 0u;  // bf16 1.0 pairs
+        Av[0] = Av[1] = Av[2] = Av[3] = one;
       }
-      if (g == 0) accb += d;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const u32x4 Bv =
+            *reinterpret_cast<const u32x4*>(dzt + (32 * h + lr) * IW_DS + wave * 64 + px0);
+        acc[h] = mfma_32x32x16(Av, Bv, acc[h]);
+      }
     }
   }
-  float* pp = part + (size_t)blockIdx.x * PART_IN;
+  // sum the four waves' accumulators (through the dz0 tile's LDS) and write
+  __syncthreads();
+  float* red = reinterpret_cast<float*>(dzt);  // 4 x 2 x 16 x 64 floats = 32 KB
 #pragma unroll
-  for (int n = 0; n < 7; ++n) {
-    const int k = g + 4 * n;
-    if (k < 27) pp[k * C + co] = acc[n];
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int q = 0; q < 16; ++q) red[((wave * 2 + h) * 16 + q) * 64 + l] = acc[h][q];
+  __syncthreads();
+  float* pp = part + (size_t)blockIdx.x * PART_IN;
+  for (int e = tid; e < 2 * 16 * 64; e += 256) {
+    const int ln = e & 63, q = (e >> 6) & 15, h = e >> 10;
+    const float v = ((red[((0 * 2 + h) * 16 + q) * 64 + ln] + red[((1 * 2 + h) * 16 + q) * 64 + ln]) +
+                     red[((2 * 2 + h) * 16 + q) * 64 + ln]) +
+                    red[((3 * 2 + h) * 16 + q) * 64 + ln];
+    const int row = 8 * (q >> 2) + 4 * (ln >> 5) + (q & 3), co = 32 * h + (ln & 31);
+    if (row < 27)
+      pp[row * C + co] = v;
+    else if (row == 27)
+      pp[NW0 + co] = v;
   }
-  if (g == 0) pp[NW0 + co] = accb;
 }
 
 // --------------------------------------------------- partial reduction

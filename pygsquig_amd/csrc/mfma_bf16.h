@@ -1,4 +1,4 @@
-// mfma_bf16.h -- bf16 storage helpers and the gfx950 bf16 MFMA used by the
+// mfma_bf16.h -- bf16 conversion and the gfx950 bf16 MFMA used by the
 // closure network's convolution kernels.
 //
 // v_mfma_f32_32x32x16_bf16: D(32x32, f32) += A(32x16, bf16) x B(16x32, bf16).
@@ -20,4 +20,14 @@ __device__ __forceinline__ f32x16 mfma_32x32x16(u32x4 a, u32x4 b, f32x16 c) {
                                                        __builtin_bit_cast(bf16x8_v, b),
                                                        __builtin_bit_cast(f32x16_v, c), 0, 0, 0);
   return __builtin_bit_cast(f32x16, r);
+}
+
+// Two floats -> packed bfloat16 pair (low half first), round to nearest
+// even: one v_cvt_pk_bf16_f32.
+__device__ __forceinline__ uint32_t bf16x2_rne(float lo, float hi) {
+  typedef __bf16 bf16x2_v __attribute__((ext_vector_type(2)));
+  bf16x2_v v;
+  v[0] = (__bf16)lo;
+  v[1] = (__bf16)hi;
+  return __builtin_bit_cast(uint32_t, v);
 }
