@@ -100,7 +100,18 @@ def main():
     ap.add_argument("--train-steps", type=int, default=1500)
     ap.add_argument("--t-spin", type=float, default=0.5)
     ap.add_argument("--t-eval", type=float, default=10.0)
-    ap.add_argument("--dt-coarse", type=float, default=5e-4)
+    ap.add_argument("--dt-coarse", default="1e-4",
+                    help="coarse time step; a comma list evaluates each (the first is reported)")
+    ap.add_argument("--nu", type=float, default=None,
+                    help="hyperviscosity (default 1e-8 * (512 / nf)^8)")
+    ap.add_argument("--coarse-seeds", default="43",
+                    help="forcing seeds of the coarse runs; extra seeds add unclosed runs that "
+                         "measure the spread between forcing realizations")
+    ap.add_argument("--no-match-forcing", dest="match_forcing", action="store_false",
+                    help="coarse forcing amplitude sqrt(eps) instead of sqrt(eps) (nc/nf)^2")
+    ap.add_argument("--closure-every", default="1",
+                    help="closure evaluated every M steps (0: every RK4 stage); a comma list "
+                         "evaluates each (the first is reported)")
     args = ap.parse_args()
     if args.copy_only:
         import torch
@@ -356,8 +367,9 @@ def bench_closure(args):
     dev = torch.device("cuda:0")
     L, alpha = 2 * np.pi, 1.0
     Nf, Nc = args.nf, args.nc
-    dt_f, dt_c = 1e-4, args.dt_coarse
-    nu = 1e-8 * (512 / Nf) ** 8
+    dt_list = [float(x) for x in str(args.dt_coarse).split(",")]
+    dt_f, dt_c = 1e-4, dt_list[0]
+    nu = 1e-8 * (512 / Nf) ** 8 if args.nu is None else args.nu
     kf, dk, eps, mu = 40.0, 2.0, 0.1, 0.1
     t_start = time.perf_counter()
 
@@ -428,18 +440,28 @@ def bench_closure(args):
     sample_dt = max(t_eval / 100, 0.01)
     n_samp = int(round(t_eval / sample_dt))
     nf_per = int(round(sample_dt / dt_f))
-    nc_per = int(round(sample_dt / dt_c))
     thb0 = C.spectral_filter(st["theta_hat"], gc)
-    forcing_c = pg.RingForcing(kf=kf, dk=dk, epsilon=eps, seed=43)
-    closed = C.ClosedSolver(gc, alpha, 0.0, 8, net=net, in_scale=in_scale,
-                            out_scale=1.0 / tgt_scale, scheme="if")
+    seeds = [int(x) for x in str(args.coarse_seeds).split(",")]
+    # The ring forcing's spectral amplitude is unnormalised (sqrt(eps) per mode), so
+    # its physical field, the rate <theta f> and hence the reference's
+    # |rate| > 1e-10 rescale threshold scale as 1/N^2.  The coarse forcing
+    # takes amplitude sqrt(eps) (Nc/Nf)^2: the same physical forcing field and
+    # threshold as the DNS (only the random phases differ).
+    amp_c = np.sqrt(eps) * (Nc / Nf) ** 2 if args.match_forcing else None
+    forcing_c = pg.RingForcing(kf=kf, dk=dk, epsilon=eps, seed=seeds[0], amplitude=amp_c)
+    every_list = [int(x) for x in str(args.closure_every).split(",")]
+    closed_by = {m: C.ClosedSolver(gc, alpha, 0.0, 8, net=net, in_scale=in_scale,
+                                   out_scale=1.0 / tgt_scale, scheme="if", every=m)
+                 for m in every_list}
     plain = pg.gSQGSolver(gc, alpha, 0.0, 8, scheme="if")
     kmax = int(Nc // 3)
 
     def shell(th_c):
         return pg.compute_shell_spectrum(th_c, gc, alpha)[: kmax + 1]
 
-    def run_coarse(solver):
+    def run_coarse(solver, dtc, forcing=None):
+        forcing = forcing_c if forcing is None else forcing
+        nc_per = int(round(sample_dt / dtc))
         s = solver.initialize()
         s = {"theta_hat": thb0.clone(), "time": s["time"], "step": s["step"]}
         acc = np.zeros(kmax + 1)
@@ -447,16 +469,25 @@ def bench_closure(args):
         for i in range(n_samp):
             sync()
             t0 = time.perf_counter()
-            s = solver.multistep(s, nc_per, dt_c, forcing=forcing_c, damping=damping)
+            s = solver.multistep(s, nc_per, dtc, forcing=forcing, damping=damping)
             sync()
             sec += time.perf_counter() - t0
             acc += shell(s["theta_hat"])
         return acc / n_samp, sec, bool(torch.isfinite(torch.view_as_real(s["theta_hat"])).all())
 
-    E_cl, sec_cl, fin_cl = run_coarse(closed)
-    log(f"closed N={Nc}: {sec_cl:.2f} s for {t_eval} time units")
-    E_pl, sec_pl, fin_pl = run_coarse(plain)
-    log(f"unclosed N={Nc}: {sec_pl:.2f} s")
+    runs = {}
+    for dtc in dt_list:
+        unclosed = run_coarse(plain, dtc)
+        for m in every_list:
+            runs[(dtc, m)] = (run_coarse(closed_by[m], dtc), unclosed)
+            log(f"dt={dtc:g} every={m}: closed N={Nc} {runs[(dtc, m)][0][1]:.2f} s, unclosed "
+                f"{unclosed[1]:.2f} s for {t_eval} time units")
+    (E_cl, sec_cl, fin_cl), (E_pl, sec_pl, fin_pl) = runs[(dt_c, every_list[0])]
+    spread = {}
+    for sd in seeds[1:]:
+        fo = pg.RingForcing(kf=kf, dk=dk, epsilon=eps, seed=sd, amplitude=amp_c)
+        spread[sd] = run_coarse(plain, dt_c, fo)[0]
+        log(f"unclosed N={Nc} with forcing seed {sd} done")
     dns_sec = 0.0
     E_dns = np.zeros(kmax + 1)
     for i in range(n_samp):
@@ -467,27 +498,42 @@ def bench_closure(args):
     E_dns /= n_samp
     dns_eval_sec = dns_sec
     ks = np.arange(1, kmax + 1)
-    err_cl = np.abs(E_cl[ks] - E_dns[ks]) / E_dns[ks]
-    err_pl = np.abs(E_pl[ks] - E_dns[ks]) / E_dns[ks]
     speedup = dns_eval_sec / sec_cl
+    def errs(E):
+        r = np.abs(E[ks] - E_dns[ks]) / E_dns[ks]
+        return {"mean_rel": float(r.mean()), "median_rel": float(np.median(r)),
+                "max_rel": float(r.max()), "within_5pct_frac": float(np.mean(r <= 0.05)),
+                "energy_weighted_rel": float(np.sum(np.abs(E[ks] - E_dns[ks])) / np.sum(E_dns[ks])),
+                "mean_abs_log_ratio": float(np.mean(np.abs(np.log(E[ks] / E_dns[ks]))))}
+
+    by_run = {}
+    for (dtc, m), ((ec, sc, _), (ep, sp, _)) in runs.items():
+        by_run[f"dt={dtc:g},every={m}"] = {"closed": errs(ec), "unclosed": errs(ep),
+                                           "closed_s": sc, "unclosed_s": sp,
+                                           "speedup": dns_eval_sec / sc,
+                                           "unclosed_speedup": dns_eval_sec / sp,
+                                           "spectrum_closed": ec[ks].tolist()}
     out = {
         "metric": "closure speedup at matched E(k) error", "value": speedup,
         "unit": "x (filtered-DNS wall-clock / closed-run wall-clock, same simulated time)",
-        "n_gpus": 1, "steps": n_samp * nc_per, "warmup": 0,
-        "ms_per_step": sec_cl / (n_samp * nc_per) * 1e3, "higher_is_better": True,
+        "n_gpus": 1, "steps": n_samp * int(round(sample_dt / dt_c)), "warmup": 0,
+        "ms_per_step": sec_cl / (n_samp * int(round(sample_dt / dt_c))) * 1e3,
+        "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64 solver, bf16/fp32 network",
         "data": "synthetic (the solver's own forced DNS)",
         "config": {"workload": f"learned closure: DNS N={Nf} -> N={Nc}", "Nf": Nf, "Nc": Nc,
                    "alpha": alpha, "dt_dns": dt_f, "dt_coarse": dt_c, "nu_p": nu,
                    "snapshots": store.count, "snap_every": args.snap_every,
                    "train_steps": args.train_steps, "t_spin": args.t_spin, "t_eval": t_eval,
+                   "coarse_forcing_amplitude": amp_c if amp_c is not None else float(np.sqrt(eps)),
                    "finite": fin_cl and fin_pl},
-        "spectrum_error": {
-            "k_max": kmax,
-            "closed_mean_rel": float(err_cl.mean()), "closed_max_rel": float(err_cl.max()),
-            "unclosed_mean_rel": float(err_pl.mean()), "unclosed_max_rel": float(err_pl.max()),
-            "closed_within_5pct_frac": float(np.mean(err_cl <= 0.05)),
-            "unclosed_within_5pct_frac": float(np.mean(err_pl <= 0.05))},
+        "spectrum_error": {"k_max": kmax, "closed": errs(E_cl), "unclosed": errs(E_pl),
+                           "closure_every": every_list[0]},
+        "runs": by_run,
+        "seed_spread": {str(sd): {"vs_dns": errs(E), "vs_unclosed_seed0": {
+            "energy_weighted_rel": float(np.sum(np.abs(E[ks] - E_pl[ks])) / np.sum(E_pl[ks])),
+            "median_rel": float(np.median(np.abs(E[ks] - E_pl[ks]) / E_pl[ks]))}}
+            for sd, E in spread.items()},
         "wall_clock_s": {"dns_eval": dns_eval_sec, "closed": sec_cl, "unclosed": sec_pl,
                          "training": train_sec},
         "training": {"loss_first": hist[0][1], "loss_last": hist[-1][1],

@@ -85,15 +85,18 @@ int gsqg_cnn_train_step(gsqg_cnn* net, const gsqg_cnn_batch* batch, double lr, d
 int gsqg_spectral_filter(const void* in_dev, int nf, void* out_dev, int nc, double scale,
                          const uint8_t* mask_dev, const void* add_dev, void* stream);
 
-/* Close a plan's dynamics with the network: every RK4 stage computes
- * (theta, u, v) of the stage state, evaluates tau = out_scale * net(...)
- * with inputs scaled by in_scale[3], and adds tau to the physical Jacobian
- * before its dealiased transform, so the stage right-hand side gains
- * -mask * fft2(tau).  The network must be n x n (the plan's grid), the plan
- * a single realization.  net == NULL detaches.  The network must outlive
- * the attachment; training it further changes the closure in place. */
+/* Close a plan's dynamics with the network: tau = out_scale * net(theta,
+ * u, v) of the stage state (inputs scaled by in_scale[3]) is added to the
+ * physical Jacobian before its dealiased transform, so the stage right-hand
+ * side gains -mask * fft2(tau).  every == 0 evaluates the network in every
+ * RK4 stage; every == M (1, 2, 4 or 8) evaluates it at the first stage of
+ * steps 0, 1, 1 + M, 1 + 2M, ... of each gsqg_advance call (gsqg_rhs:
+ * always) and holds tau until the next evaluation.  The network must be n x n
+ * (the plan's grid), the plan a single realization.  net == NULL detaches.
+ * The network must outlive the attachment; training it further changes the
+ * closure in place. */
 int gsqg_plan_attach_closure(gsqg_plan* plan, const gsqg_cnn* net, const double* in_scale,
-                             double out_scale);
+                             double out_scale, int every);
 
 #ifdef __cplusplus
 }

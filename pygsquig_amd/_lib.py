@@ -124,7 +124,7 @@ _SIGS = {
     "gsqg_cnn_train_step": (ctypes.c_int, [_VP, ctypes.POINTER(GsqgCnnBatch), ctypes.c_double,
                                            ctypes.POINTER(ctypes.c_double), _VP]),
     "gsqg_plan_attach_closure": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(ctypes.c_double),
-                                                ctypes.c_double]),
+                                                ctypes.c_double, ctypes.c_int]),
     "gsqg_spectral_filter": (ctypes.c_int, [_VP, ctypes.c_int, _VP, ctypes.c_int, ctypes.c_double,
                                             _VP, _VP, _VP]),
 }

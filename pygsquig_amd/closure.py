@@ -292,32 +292,36 @@ def train(net: ClosureNet, store: SnapshotStore, steps: int, batch: int = 32, lr
 
 
 class ClosedSolver(gSQGSolver):
-    """gSQGSolver whose every plan carries the learned closure: each RK4
-    stage adds tau = out_scale * net(theta * s0, u * s1, v * s2) to the
-    physical Jacobian before its dealiased transform."""
+    """gSQGSolver whose every plan carries the learned closure:
+    tau = out_scale * net(theta * s0, u * s1, v * s2) is added to the
+    physical Jacobian before its dealiased transform, fresh in every RK4
+    stage (``every=0``) or from the first stage of every ``every``-th step
+    (1, 2, 4 or 8) and held in between."""
 
     def __init__(self, grid: Grid, alpha: float, nu_p: float = 0.0, p: int = 8, *,
                  net: ClosureNet, in_scale: Sequence[float], out_scale: float,
-                 scheme: str = "if"):
+                 scheme: str = "if", every: int = 0):
         super().__init__(grid, alpha, nu_p, p, scheme=scheme)
         if net.h != grid.N or net.w != grid.N:
             raise ValueError(f"network is {net.h} x {net.w}, grid is {grid.N} x {grid.N}")
         self.net = net
         self.in_scale = tuple(float(s) for s in in_scale)
         self.out_scale = float(out_scale)
+        self.every = int(every)
 
     def _plan(self, forcing, damping, combine=None, kappas=None) -> Plan:
         plan = super()._plan(forcing, damping, combine, kappas)
         if getattr(plan, "closure", None) is not self.net:
-            attach_closure(plan, self.net, self.in_scale, self.out_scale)
+            attach_closure(plan, self.net, self.in_scale, self.out_scale, self.every)
         return plan
 
 
 def attach_closure(plan: Plan, net: Optional[ClosureNet], in_scale=(1.0, 1.0, 1.0),
-                   out_scale: float = 1.0) -> None:
+                   out_scale: float = 1.0, every: int = 0) -> None:
     """Attach (or with ``net=None`` detach) the closure to a plan."""
     lib = _lib.load()
     sc = (ctypes.c_double * 3)(*[float(s) for s in in_scale])
     _lib.check(lib.gsqg_plan_attach_closure(plan.handle, net.handle if net is not None else None,
-                                            sc, float(out_scale)), "gsqg_plan_attach_closure")
+                                            sc, float(out_scale), int(every)),
+               "gsqg_plan_attach_closure")
     plan.closure = net  # keeps the network alive while attached

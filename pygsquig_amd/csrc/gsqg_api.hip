@@ -122,6 +122,7 @@ struct gsqg_plan {
   const gsqg_cnn* cl_net = nullptr;
   double cl_in[3] = {1.0, 1.0, 1.0};
   double cl_out = 1.0;
+  int cl_every = 0;  // 0: tau in every stage; M: from the first stage of every M-th step
   double2 *cl_ys = nullptr, *cl_f = nullptr;
   double *cl_th = nullptr, *cl_u = nullptr, *cl_v = nullptr;
   uint16_t *cl_a0 = nullptr, *cl_a1 = nullptr;
@@ -444,10 +445,14 @@ static int closure_tau(gsqg_plan* p, hipStream_t st) {
 }
 
 // Row pass of a theta stage (with the closure flux when one is attached).
-static int theta_row(gsqg_plan* p, RKArgs a, int ny, hipStream_t st) {
+// `fresh` asks for a new tau at this stage: the first stage of a step that
+// evaluates the network; every stage when tau is per stage.
+static int theta_row(gsqg_plan* p, RKArgs a, int ny, hipStream_t st, bool fresh) {
   if (p->cl_net) {
-    int rc = closure_tau(p, st);
-    if (rc) return rc;
+    if (fresh || p->cl_every == 0) {
+      int rc = closure_tau(p, st);
+      if (rc) return rc;
+    }
     a.tau = p->cl_tau;
     HIPCHK(rk(p, RK_THETA_CL, a, 1, st));
     return GSQG_OK;
@@ -532,7 +537,7 @@ static int theta_rest(gsqg_plan* p, double dt, bool derived, bool last_prologue,
   const int N = p->N;
   for (int e = 1; e <= 3; ++e) {
     HIPCHK(ck(p, CK_THETA, ck_theta(p, p->th, e, 1, e + 1, 0, dt), p->n_mem, st));
-    int rc = theta_row(p, rk_theta(p, e + 1, 0, false), p->n_mem, st);
+    int rc = theta_row(p, rk_theta(p, e + 1, 0, false), p->n_mem, st, false);
     if (rc) return rc;
   }
   HIPCHK(ck(p, CK_THETA,
@@ -551,8 +556,8 @@ static int theta_rest(gsqg_plan* p, double dt, bool derived, bool last_prologue,
 // pass, which rewrites the velocity, is ordered after the join.
 static int step_body(gsqg_plan* p, double dt, bool derived, bool scalars_pending,
                      bool last_prologue, int last_rate, bool scalar_next_prologue,
-                     hipStream_t st, hipStream_t side = nullptr) {
-  int rc = theta_row(p, rk_theta(p, 1, derived ? 1 : 0, scalars_pending), p->n_mem, st);
+                     hipStream_t st, hipStream_t side = nullptr, bool cl_eval = true) {
+  int rc = theta_row(p, rk_theta(p, 1, derived ? 1 : 0, scalars_pending), p->n_mem, st, cl_eval);
   if (rc) return rc;
   if (scalars_pending && side) {
     HIPCHK(hipEventRecord(p->ev_fork, st));
@@ -569,6 +574,13 @@ static int step_body(gsqg_plan* p, double dt, bool derived, bool scalars_pending
   return theta_rest(p, dt, derived, last_prologue, last_rate, st);
 }
 
+// Whether middle step j (j = step index - 1 within a gsqg_advance call)
+// evaluates the closure network: every M-th step, M dividing kGraphSteps,
+// so the captured multi-step graph repeats the pattern.
+static bool cl_step_evals(const gsqg_plan* p, int j) {
+  return p->cl_every <= 1 || (j % p->cl_every) == 0;
+}
+
 // Capture nsteps steady-state step bodies into one executable graph.
 static int capture_steps(gsqg_plan* p, int nsteps, double dt, bool derived, bool sc,
                          hipGraphExec_t* out) {
@@ -576,7 +588,7 @@ static int capture_steps(gsqg_plan* p, int nsteps, double dt, bool derived, bool
   HIPCHK(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
   int rc = GSQG_OK;
   for (int s = 0; s < nsteps && rc == GSQG_OK; ++s)
-    rc = step_body(p, dt, derived, sc, true, 1, true, p->cap, p->cap2);
+    rc = step_body(p, dt, derived, sc, true, 1, true, p->cap, p->cap2, cl_step_evals(p, s));
   hipError_t ce = hipStreamEndCapture(p->cap, &g);
   if (rc) return rc;
   if (ce != hipSuccess) return fail(GSQG_ERR_HIP, std::string("capture: ") + hipGetErrorString(ce));
@@ -606,7 +618,7 @@ extern "C" int gsqg_advance(gsqg_plan* p, double dt, int n_steps, int key_mode, 
   // step 0
   {
     const bool last = (n_steps == 1);
-    if ((rc = theta_row(p, rk_theta(p, 1, derived ? 1 : 0, false), p->n_mem, st))) return rc;
+    if ((rc = theta_row(p, rk_theta(p, 1, derived ? 1 : 0, false), p->n_mem, st, true))) return rc;
     if ((rc = theta_rest(p, dt, derived, !last || sc, last ? 0 : 1, st))) return rc;
   }
   // middle steps 1 .. n-2
@@ -622,21 +634,33 @@ extern "C" int gsqg_advance(gsqg_plan* p, double dt, int n_steps, int key_mode, 
         p->g_key = key;
       }
       const int nk = n_mid / kGraphSteps, n1 = n_mid % kGraphSteps;
+      // a one-step graph repeats one evaluation pattern: with a closure held
+      // over several steps the remainder runs eagerly
+      const bool graph1 = !(p->cl_net && p->cl_every > 1);
       if (nk > 0 && !p->gexec_k)
         if ((rc = capture_steps(p, kGraphSteps, dt, derived, sc, &p->gexec_k))) return rc;
-      if (n1 > 0 && !p->gexec)
+      if (n1 > 0 && graph1 && !p->gexec)
         if ((rc = capture_steps(p, 1, dt, derived, sc, &p->gexec))) return rc;
       for (int s = 0; s < nk; ++s) HIPCHK(hipGraphLaunch(p->gexec_k, st));
-      for (int s = 0; s < n1; ++s) HIPCHK(hipGraphLaunch(p->gexec, st));
+      for (int s = 0; s < n1; ++s) {
+        if (graph1) {
+          HIPCHK(hipGraphLaunch(p->gexec, st));
+        } else if ((rc = step_body(p, dt, derived, sc, true, 1, true, st, nullptr,
+                                   cl_step_evals(p, s)))) {
+          return rc;
+        }
+      }
     } else {
       for (int s = 0; s < n_mid; ++s) {
-        if ((rc = step_body(p, dt, derived, sc, true, 1, true, st))) return rc;
+        if ((rc = step_body(p, dt, derived, sc, true, 1, true, st, nullptr, cl_step_evals(p, s))))
+          return rc;
       }
     }
   }
   // last step (if n >= 2)
   if (n_steps >= 2) {
-    if ((rc = step_body(p, dt, derived, sc, sc, 0, true, st))) return rc;
+    if ((rc = step_body(p, dt, derived, sc, sc, 0, true, st, nullptr, cl_step_evals(p, n_mid))))
+      return rc;
   }
   if (sc) {
     // velocity of the final theta (prologue fields F2, F3 of the last CK)
@@ -746,7 +770,7 @@ extern "C" int gsqg_rhs(gsqg_plan* p, const void* X, void* out, uint64_t key, vo
   CKArgs c0 = ck_theta(p, p->th2, 0, 1, 1, 0, 0.0);
   c0.ph.scheme = 0;
   HIPCHK(ck(p, CK_THETA, c0, 1, st));
-  int rc = theta_row(p, rk_theta(p, 1, 0, false), 1, st);
+  int rc = theta_row(p, rk_theta(p, 1, 0, false), 1, st, true);
   if (rc) return rc;
   // Stage-1 epilogue writes k1 into acc (th2 is left untouched for e == 1).
   CKArgs c1 = ck_theta(p, p->th2, 1, 0, 0, 0, 0.0);
@@ -1129,7 +1153,7 @@ extern "C" int gsqg_cnn_train_step(gsqg_cnn* n, const gsqg_cnn_batch* b, double 
 }
 
 extern "C" int gsqg_plan_attach_closure(gsqg_plan* p, const gsqg_cnn* net, const double* in_scale,
-                                        double out_scale) {
+                                        double out_scale, int every) {
   if (!p) return fail(GSQG_ERR_ARG, "null plan");
   // captured graphs hold the previous schedule
   if (p->gexec) HIPCHK(hipGraphExecDestroy(p->gexec));
@@ -1141,6 +1165,9 @@ extern "C" int gsqg_plan_attach_closure(gsqg_plan* p, const gsqg_cnn* net, const
     return GSQG_OK;
   }
   if (!in_scale) return fail(GSQG_ERR_ARG, "null input scales");
+  if (every < 0 || every > kGraphSteps || (every > 0 && kGraphSteps % every != 0))
+    return fail(GSQG_ERR_ARG, "every must be 0 (per stage) or divide " +
+                                  std::to_string(kGraphSteps) + ", got " + std::to_string(every));
   const int N = p->N;
   if (net->H != N || net->W != N)
     return fail(GSQG_ERR_ARG, "the network is " + std::to_string(net->H) + " x " +
@@ -1161,6 +1188,7 @@ extern "C" int gsqg_plan_attach_closure(gsqg_plan* p, const gsqg_cnn* net, const
   p->cl_net = net;
   for (int c = 0; c < 3; ++c) p->cl_in[c] = in_scale[c];
   p->cl_out = out_scale;
+  p->cl_every = every;
   return GSQG_OK;
 }
 
