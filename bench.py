@@ -91,7 +91,7 @@ def main():
     ap.add_argument("--copy-only", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates PMC byte counters)")
     ap.add_argument("--config", default="forced",
-                    choices=["forced", "plain", "large", "batch", "closure"])
+                    choices=["forced", "plain", "large", "batch", "closure", "cnn"])
     # closure workload sizes
     ap.add_argument("--nf", type=int, default=4096)
     ap.add_argument("--nc", type=int, default=512)
@@ -132,6 +132,8 @@ def main():
         return bench_batch(args)
     if args.config == "closure":
         return bench_closure(args)
+    if args.config == "cnn":
+        return bench_cnn(args)
     if args.config in ("plain", "large"):
         return bench_single(args)
 
@@ -353,6 +355,47 @@ def bench_batch(args):
                           "steps": loop_steps, "speedup": value / loop_value},
         "cpu_baseline": None,
         "note": "cpu_baseline is reported for the default configuration only"}))
+
+
+def bench_cnn(args):
+    """The closure network alone on nc x nc images: the forward pass of one
+    image (what a closed solver step evaluates) and training steps on a
+    batch of 32 random fields."""
+    import torch
+
+    from pygsquig_amd.closure import ClosureNet
+
+    dev = torch.device("cuda:0")
+    H = args.nc
+    B = 32
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    th, u, v, tg = (torch.randn((B + 1, H, H), dtype=torch.float64, device=dev, generator=g)
+                    for _ in range(4))
+    net = ClosureNet(H, H, max_batch=B, device=dev, seed=0)
+    sel = torch.arange(B, dtype=torch.int32, device=dev)
+    for _ in range(3):
+        net.forward(th[:1], u[:1], v[:1], (1, 1, 1))
+    n_fwd = 20
+    _, sf = _timed(lambda: [net.forward(th[:1], u[:1], v[:1], (1, 1, 1)) for _ in range(n_fwd)],
+                   n_fwd)
+    net.train_step(th, u, v, tg, (1, 1, 1), 1.0, sel=sel)
+    n_tr = 5
+    _, st = _timed(lambda: [net.train_step(th, u, v, tg, (1, 1, 1), 1.0, sel=sel)
+                            for _ in range(n_tr)], n_tr)
+    flop = 2.0 * H * H * (27 * 64 + 4 * 576 * 64 + 576)
+    fwd_tf = flop * n_fwd / sf / 1e12
+    tr_tf = 3 * flop * B * n_tr / st / 1e12
+    print(json.dumps({
+        "metric": "closure network forward and training throughput", "value": fwd_tf,
+        "unit": "TFLOP/s (forward, one image)", "n_gpus": 1, "steps": n_fwd, "warmup": 3,
+        "ms_per_step": sf / n_fwd * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 (fp32 accumulation)", "data": "synthetic",
+        "config": {"workload": f"6-layer 3x3 CNN, 64 channels, {H}x{H}", "batch_train": B},
+        "roofline": {"bound": "mfma", "achieved": fwd_tf, "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": fwd_tf / 2500.0, "traffic": None},
+        "training": {"ms_per_step": st / n_tr * 1e3, "tflops": tr_tf, "frac": tr_tf / 2500.0},
+        "cpu_baseline": None}))
 
 
 def bench_closure(args):

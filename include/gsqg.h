@@ -128,7 +128,7 @@ int gsqg_store_scalar(gsqg_plan* plan, int idx, void* scalar_hat_dev, void* stre
  * key_mode GSQG_KEY_EXPLICIT: `key` drives the forcing phases of every step
  * (the reference passes one key per step); GSQG_KEY_DERIVED: step s uses
  * step_key(key, step0 + s).  use_graph != 0 captures the steps between the
- * first and the last in HIP graphs (eight steps per graph, plus a one-step
+ * first and the last in HIP graphs (16 steps per graph, plus a one-step
  * graph for the remainder) and replays them; the graphs are kept until dt,
  * the key mode or the key changes. */
 int gsqg_advance(gsqg_plan* plan, double dt, int n_steps, int key_mode, uint64_t key,
@@ -197,6 +197,15 @@ int gsqg_spectral_diagnostics(gsqg_plan* plan, const void* spectrum_dev, double 
  * 3 host doubles; synchronises `stream`. */
 int gsqg_physical_diagnostics(gsqg_plan* plan, const void* spectrum_dev, double* out,
                               void* stream);
+
+/* CFL controller quantities of a full (n, n) complex128 spectrum X:
+ * out[0] = max sqrt(u^2 + v^2) of its velocity (compute_max_velocity,
+ * core/adaptive_timestep.py:57-70; compute_velocity_from_theta,
+ * core/adaptive_solver.py:106), out[1] = max |X| (the growth check of
+ * AdaptiveTimestepper.check_stability, core/adaptive_timestep.py:256-262).
+ * A NaN anywhere makes the maximum NaN.  out is 2 host doubles;
+ * synchronises `stream`. */
+int gsqg_cfl_extrema(gsqg_plan* plan, const void* spectrum_dev, double* out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -89,7 +89,7 @@ int gsqg_spectral_filter(const void* in_dev, int nf, void* out_dev, int nc, doub
  * u, v) of the stage state (inputs scaled by in_scale[3]) is added to the
  * physical Jacobian before its dealiased transform, so the stage right-hand
  * side gains -mask * fft2(tau).  every == 0 evaluates the network in every
- * RK4 stage; every == M (1, 2, 4 or 8) evaluates it at the first stage of
+ * RK4 stage; every == M (1, 2, 4, 8 or 16) evaluates it at the first stage of
  * steps 0, 1, 1 + M, 1 + 2M, ... of each gsqg_advance call (gsqg_rhs:
  * always) and holds tau until the next evaluation.  The network must be n x n
  * (the plan's grid), the plan a single realization.  net == NULL detaches.

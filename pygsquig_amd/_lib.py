@@ -109,6 +109,7 @@ _SIGS = {
                                                  ctypes.POINTER(ctypes.c_double),
                                                  ctypes.POINTER(ctypes.c_double), _VP]),
     "gsqg_physical_diagnostics": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(ctypes.c_double), _VP]),
+    "gsqg_cfl_extrema": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(ctypes.c_double), _VP]),
     # include/gsqg_closure.h
     "gsqg_cnn_num_params": (ctypes.c_int, []),
     "gsqg_cnn_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,

@@ -296,7 +296,7 @@ class ClosedSolver(gSQGSolver):
     tau = out_scale * net(theta * s0, u * s1, v * s2) is added to the
     physical Jacobian before its dealiased transform, fresh in every RK4
     stage (``every=0``) or from the first stage of every ``every``-th step
-    (1, 2, 4 or 8) and held in between."""
+    (1, 2, 4, 8 or 16) and held in between."""
 
     def __init__(self, grid: Grid, alpha: float, nu_p: float = 0.0, p: int = 8, *,
                  net: ClosureNet, in_scale: Sequence[float], out_scale: float,

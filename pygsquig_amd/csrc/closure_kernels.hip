@@ -102,50 +102,70 @@ __device__ __forceinline__ void stage_tile(uint16_t* li, const uint16_t* __restr
 }
 
 // ------------------------------------------------------- layer 0 forward
-// z0 = bf16(conv(x, W0r) + b0), a0 = bf16(gelu(z0)), x = bf16(plane * scale).
+// z0 = bf16(conv(x, W0r) + b0), a0 = bf16(gelu(z0)), x = bf16(plane * scale):
+// one MFMA GEMM per wave, 64 output channels x the 64 pixels of one tile
+// row, the reduction over the 27 (tap, channel) pairs k = 3 t + ci, padded
+// to 32 with zeros.  The forward epilogue (shared with the 64-channel
+// layers) is defined below.
+__device__ __forceinline__ void store_fwd(const f32x16 (&acc)[2][2], const float* __restrict__ bias,
+                                          uint16_t* zout, uint16_t* aout, size_t prow, int lr,
+                                          int lh);
+
 __global__ void __launch_bounds__(256) conv_in_fwd(Feat f, const float* __restrict__ W0r,
                                                    const float* __restrict__ b0, uint16_t* z,
                                                    uint16_t* a, int H, int W) {
-  __shared__ float xs[3][HR][HC];
-  __shared__ float ws[NW0 + C];
-  const int tid = threadIdx.x;
+  __shared__ uint16_t xs[3][HR][HC];
+  __shared__ __attribute__((aligned(16))) uint16_t wt[C * 32];  // [co][k]
+  const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63, lr = l & 31, lh = l >> 5;
   const int j0 = blockIdx.x * TC, i0 = blockIdx.y * TR, b = blockIdx.z;
   const long long img = f.sel ? f.sel[b] : b;
   for (int e = tid; e < 3 * HR * HC; e += 256) {
     const int c = e / (HR * HC), q = e - c * (HR * HC), r = q / HC, cc = q - r * HC;
     const int gi = wrap(i0 - 1 + r, H), gj = wrap(j0 - 1 + cc, W);
     const double v = f.plane[c][img * f.img_stride + (long long)gi * W + gj] * f.in_scale[c];
-    xs[c][r][cc] = rbf((float)v);
+    xs[c][r][cc] = (uint16_t)f2bf((float)v);
   }
-  for (int e = tid; e < NW0; e += 256) ws[e] = W0r[e];
-  if (tid < C) ws[NW0 + tid] = b0[tid];
+  for (int e = tid; e < C * 32; e += 256) {
+    const int co = e >> 5, k = e & 31;
+    wt[e] = k < 27 ? (uint16_t)f2bf(W0r[k * C + co]) : (uint16_t)0;
+  }
   __syncthreads();
-  const int r = tid >> 6, c = tid & 63;
-  float acc[C];
+  f32x16 acc[2][2];
 #pragma unroll
-  for (int co = 0; co < C; ++co) acc[co] = 0.0f;
-  for (int t = 0; t < 9; ++t) {
-    const int dy = t / 3 - 1, dx = t % 3 - 1;
+  for (int h = 0; h < 2; ++h)
 #pragma unroll
-    for (int ci = 0; ci < 3; ++ci) {
-      const float x = xs[ci][r + 1 + dy][c + 1 + dx];
-      const float* w = ws + (t * 3 + ci) * C;
+    for (int g = 0; g < 2; ++g)
 #pragma unroll
-      for (int co = 0; co < C; ++co) acc[co] += x * w[co];
+      for (int r = 0; r < 16; ++r) acc[h][g][r] = 0.0f;
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks) {
+    u32x4 Bf[2];
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+#pragma unroll
+      for (int w2 = 0; w2 < 4; ++w2) {
+        uint32_t word = 0;
+#pragma unroll
+        for (int hh = 0; hh < 2; ++hh) {
+          const int k = 16 * ks + 8 * lh + 2 * w2 + hh;
+          uint32_t v = 0;
+          if (k < 27) {
+            const int t = k / 3, ci = k - 3 * t;
+            v = xs[ci][wave + 1 + (t / 3 - 1)][32 * g + lr + 1 + (t % 3 - 1)];
+          }
+          word |= v << (16 * hh);
+        }
+        Bf[g][w2] = word;
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const u32x4 A = *reinterpret_cast<const u32x4*>(wt + (32 * h + lr) * 32 + 16 * ks + 8 * lh);
+      acc[h][0] = mfma_32x32x16(A, Bf[0], acc[h][0]);
+      acc[h][1] = mfma_32x32x16(A, Bf[1], acc[h][1]);
     }
   }
-  const size_t pix = ((size_t)b * H + i0 + r) * W + j0 + c;
-#pragma unroll
-  for (int h = 0; h < 8; ++h) {
-    float zf[8], af[8];
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      zf[e] = rbf(acc[8 * h + e] + ws[NW0 + 8 * h + e]);
-      af[e] = gelu(zf[e]);
-    }
-    if (z) *reinterpret_cast<u32x4*>(z + pix * C + 8 * h) = pack8(zf);
-    *reinterpret_cast<u32x4*>(a + pix * C + 8 * h) = pack8(af);
-  }
+  store_fwd(acc, b0, z, a, ((size_t)b * H + i0 + wave) * W + j0, lr, lh);
 }
 
 // -------------------------------------------- 64 -> 64 layers (MFMA GEMM)
@@ -214,76 +234,122 @@ __global__ void __launch_bounds__(256, 2) conv_mid(const uint16_t* __restrict__ 
       }
     }
   }
-  // epilogue straight from the accumulators: lane (lr, lh) holds channels
-  // co0 .. co0 + 3 (co0 = 32 h + 8 q + 4 lh) of pixel 32 g + lr of row `wave`
+  // epilogue straight from the accumulators
   const size_t prow = (ib + i0 + wave) * W + j0;
+  if (MODE == 0) {
+    store_fwd(acc, bias, zout, aout, prow, lr, lh);
+    return;
+  }
 #pragma unroll
   for (int h = 0; h < 2; ++h)
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int co0 = 32 * h + 8 * q + 4 * lh;
-      float bb[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-      if (MODE == 0) {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) bb[e] = bias[co0 + e];
-      }
 #pragma unroll
       for (int g = 0; g < 2; ++g) {
         const size_t o = (prow + 32 * g + lr) * C + co0;
-        float v[4];
+        const uint32_t* zq = reinterpret_cast<const uint32_t*>(zprev + o);
+        const uint32_t p0 = zq[0], p1 = zq[1];
+        const f32x16& v = acc[h][g];
+        uint32_t* ap = reinterpret_cast<uint32_t*>(aout + o);
+        ap[0] = bf16x2_rne(v[4 * q] * gelu_d(bf2f(p0 & 0xffffu)),
+                           v[4 * q + 1] * gelu_d(bf2f(p0 >> 16)));
+        ap[1] = bf16x2_rne(v[4 * q + 2] * gelu_d(bf2f(p1 & 0xffffu)),
+                           v[4 * q + 3] * gelu_d(bf2f(p1 >> 16)));
+      }
+    }
+}
+
+// Forward epilogue of a 64-channel x 64-pixel wave tile (rows of `acc`:
+// channels 32 h + 8 q + 4 lh + e, columns: pixels 32 g + lr of the image row
+// starting at pixel index prow): z = bf16(acc + bias), a = bf16(gelu(z)).
+__device__ __forceinline__ void store_fwd(const f32x16 (&acc)[2][2], const float* __restrict__ bias,
+                                          uint16_t* zout, uint16_t* aout, size_t prow, int lr,
+                                          int lh) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) v[e] = acc[h][g][4 * q + e];
-        uint32_t w0, w1;
-        if (MODE == 0) {
-          const uint32_t z0 = bf16x2_rne(v[0] + bb[0], v[1] + bb[1]);
-          const uint32_t z1 = bf16x2_rne(v[2] + bb[2], v[3] + bb[3]);
-          if (zout) {
-            uint32_t* zp = reinterpret_cast<uint32_t*>(zout + o);
-            zp[0] = z0;
-            zp[1] = z1;
-          }
-          w0 = bf16x2_rne(gelu(bf2f(z0 & 0xffffu)), gelu(bf2f(z0 >> 16)));
-          w1 = bf16x2_rne(gelu(bf2f(z1 & 0xffffu)), gelu(bf2f(z1 >> 16)));
-        } else {
-          const uint32_t* zq = reinterpret_cast<const uint32_t*>(zprev + o);
-          const uint32_t p0 = zq[0], p1 = zq[1];
-          w0 = bf16x2_rne(v[0] * gelu_d(bf2f(p0 & 0xffffu)), v[1] * gelu_d(bf2f(p0 >> 16)));
-          w1 = bf16x2_rne(v[2] * gelu_d(bf2f(p1 & 0xffffu)), v[3] * gelu_d(bf2f(p1 >> 16)));
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int co0 = 32 * h + 8 * q + 4 * lh;
+      float bb[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) bb[e] = bias[co0 + e];
+#pragma unroll
+      for (int g = 0; g < 2; ++g) {
+        const size_t o = (prow + 32 * g + lr) * C + co0;
+        const f32x16& v = acc[h][g];
+        const uint32_t z0 = bf16x2_rne(v[4 * q] + bb[0], v[4 * q + 1] + bb[1]);
+        const uint32_t z1 = bf16x2_rne(v[4 * q + 2] + bb[2], v[4 * q + 3] + bb[3]);
+        if (zout) {
+          uint32_t* zp = reinterpret_cast<uint32_t*>(zout + o);
+          zp[0] = z0;
+          zp[1] = z1;
         }
         uint32_t* ap = reinterpret_cast<uint32_t*>(aout + o);
-        ap[0] = w0;
-        ap[1] = w1;
+        ap[0] = bf16x2_rne(gelu(bf2f(z0 & 0xffffu)), gelu(bf2f(z0 >> 16)));
+        ap[1] = bf16x2_rne(gelu(bf2f(z1 & 0xffffu)), gelu(bf2f(z1 >> 16)));
       }
     }
 }
 
 // --------------------------------------------------- layer 5 forward
-// tau = out_scale * (conv(a4, W5r) + b5), one pixel per thread.
+// tau = out_scale * (b5 + sum_t y[p + off_t][t]) with the per-tap partial
+// sums y[q][t] = sum_ci a4[q][ci] W5r[t][ci] of every halo pixel q: an MFMA
+// GEMM (M = pixels, N = 9 taps padded to 32, K = 64 channels) over the
+// swizzled halo tile, then a 9-term gather per output pixel.
+constexpr int NHALO = HR * HC;               // 396 halo pixels
+constexpr int NGRP = (NHALO + 31) / 32;      // 13 groups of 32
+
 __global__ void __launch_bounds__(256) conv_out_fwd(const uint16_t* __restrict__ a,
                                                     const float* __restrict__ W5r,
                                                     const float* __restrict__ P, float out_scale,
                                                     float* tau, int H, int W) {
-  __shared__ __attribute__((aligned(16))) uint16_t li[IN_E];
-  __shared__ float ws[NW5];
-  const int tid = threadIdx.x;
+  __shared__ __attribute__((aligned(16))) uint16_t li[MID_IN];
+  __shared__ __attribute__((aligned(16))) uint16_t w5[32 * C];  // rows t (zero for t >= 9)
+  __shared__ float ys[NHALO * 9];
+  const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63, lr = l & 31, lh = l >> 5;
   const int j0 = blockIdx.x * TC, i0 = blockIdx.y * TR, b = blockIdx.z;
   const size_t ib = (size_t)b * H;
-  stage_tile(li, a, ib, i0, j0, H, W, tid);
-  for (int e = tid; e < NW5; e += 256) ws[e] = W5r[e];
+  for (int e = tid; e < HR * HC * 8; e += 256) {
+    const int q = e >> 3, h = e & 7, r = q / HC, cc = q - r * HC;
+    const int gi = wrap(i0 - 1 + r, H), gj = wrap(j0 - 1 + cc, W);
+    *reinterpret_cast<u32x4*>(li + swz(q, h)) =
+        *reinterpret_cast<const u32x4*>(a + ((ib + gi) * W + gj) * C + 8 * h);
+  }
+  {
+    const int t = tid >> 3, h = tid & 7;  // 32 rows x 8 chunks = 256 threads
+    float v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = t < 9 ? W5r[t * C + 8 * h + k] : 0.0f;
+    *reinterpret_cast<u32x4*>(w5 + swz(t, h)) = pack8(v);
+  }
   __syncthreads();
-  const int r = tid >> 6, c = tid & 63;
-  float acc = 0.0f;
-  for (int t = 0; t < 9; ++t) {
-    const int dy = t / 3 - 1, dx = t % 3 - 1;
-    const uint16_t* p = li + ((r + 1 + dy) * HC + c + 1 + dx) * PS;
+  for (int grp = wave; grp < NGRP; grp += 4) {
+    const int qa = 32 * grp + lr;
+    const int qr = qa < NHALO ? qa : NHALO - 1;  // rows past the tile: read a valid pixel, drop
+    f32x16 acc;
 #pragma unroll
-    for (int h = 0; h < 8; ++h) {
-      const u32x4 v = *reinterpret_cast<const u32x4*>(p + 8 * h);
+    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
 #pragma unroll
-      for (int k = 0; k < 8; ++k) acc += unpack(v, k) * ws[t * C + 8 * h + k];
+    for (int kc = 0; kc < 4; ++kc) {
+      const u32x4 A = *reinterpret_cast<const u32x4*>(li + swz(qr, 2 * kc + lh));
+      const u32x4 B = *reinterpret_cast<const u32x4*>(w5 + swz(lr, 2 * kc + lh));
+      acc = mfma_32x32x16(A, B, acc);
+    }
+    if (lr < 9) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int q = 32 * grp + 8 * (r >> 2) + 4 * lh + (r & 3);
+        if (q < NHALO) ys[q * 9 + lr] = acc[r];
+      }
     }
   }
-  tau[(ib + i0 + r) * W + j0 + c] = out_scale * (acc + P[OFF_B5]);
+  __syncthreads();
+  const int r = tid >> 6, c = tid & 63;
+  float s = 0.0f;
+#pragma unroll
+  for (int t = 0; t < 9; ++t) s += ys[((r + 1 + t / 3 - 1) * HC + c + 1 + t % 3 - 1) * 9 + t];
+  tau[(ib + i0 + r) * W + j0 + c] = out_scale * (s + P[OFF_B5]);
 }
 
 // ------------------------------------------------------------ loss

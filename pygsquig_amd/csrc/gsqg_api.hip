@@ -34,6 +34,8 @@ hipError_t launch_phys_diag(const double* th, const double* u, const double* v, 
                             double* part, hipStream_t s);
 hipError_t launch_diag_reduce(const double* part, int nblk, int stride, int nsum, double* out,
                               hipStream_t s);
+hipError_t launch_cfl(const double* u, const double* v, const double2* X, long long n,
+                      long long nx, double* part, hipStream_t s);
 hipError_t launch_a_tab(const int2* modes, const double* k1d, const uint8_t* flags, ATab* out, int n,
                         int N, hipStream_t s);
 hipError_t launch_a_rhs(const Phys& ph, const double* k1d, const uint8_t* flags, const Ctrl* ctrl,
@@ -63,7 +65,7 @@ static int fail(int code, const std::string& msg) {
   } while (0)
 
 static const int kMaxScalars = 8;
-static const int kGraphSteps = 8;  // steps per replayed graph (amortises the launch)
+static const int kGraphSteps = 16;  // steps per replayed graph (amortises the launch)
 static const int kMaxBins = 4096;  // reference energy-spectrum bins per call
 
 struct gsqg_plan {
@@ -974,6 +976,30 @@ extern "C" int gsqg_physical_diagnostics(gsqg_plan* p, const void* X, double* ou
   out[0] = 0.5 * (h[1] * inv);
   out[1] = 0.5 * (h[0] * inv);
   out[2] = h[2];
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_cfl_extrema(gsqg_plan* p, const void* X, double* out, void* stream) {
+  if (!p || !X || !out) return fail(GSQG_ERR_ARG, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const int N = p->N;
+  int rc;
+  if ((rc = diag_scratch(p, 2))) return rc;
+  if ((rc = gsqg_velocity(p, X, p->uphys, p->vphys, stream))) return rc;
+  const long long n = (long long)N * N;
+  HIPCHK(launch_cfl(p->uphys, p->vphys, (const double2*)X, n, n, p->d_part, st));
+  const int nb = diag_blocks();
+  std::vector<double> h((size_t)nb * 2);
+  HIPCHK(hipMemcpyAsync(h.data(), p->d_part, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  double a = 0.0, b = 0.0;
+  for (int i = 0; i < nb; ++i) {
+    const double x = h[2 * (size_t)i], y = h[2 * (size_t)i + 1];
+    a = (x != x || x > a) ? x : a;
+    b = (y != y || y > b) ? y : b;
+  }
+  out[0] = a;
+  out[1] = b;
   return GSQG_OK;
 }
 

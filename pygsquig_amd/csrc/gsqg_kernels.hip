@@ -465,17 +465,24 @@ __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
     if (E > 0) sc = ctrl->scale[E];
   }
   cd y[8];
-  cd thv[8], accv[8], ysv[8];
+  // LEAN (1024-lane blocks, 128 registers per lane): the epilogue's per-mode
+  // state is loaded where it is used instead of up front, which avoids
+  // spilling at the cost of loads queued behind earlier stores.
+  constexpr bool LEAN = NT > 512;
+  constexpr int NPRE = LEAN ? 1 : 8;
+  cd thv[NPRE], accv[NPRE], ysv[NPRE];
   const bool need_ys = (ph.scheme == 0) && (E >= 3);
   if (E > 0) {
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = a.J[ro + t + m * Q];
+    if (!LEAN) {
 #pragma unroll
-    for (int m = 0; m < 8; ++m) thv[m] = a.th[ro + t + m * Q];
+      for (int m = 0; m < NPRE; ++m) thv[m] = a.th[ro + t + m * Q];
 #pragma unroll
-    for (int m = 0; m < 8; ++m) accv[m] = (E >= 2) ? a.acc[ro + t + m * Q] : cmk(0.0, 0.0);
+      for (int m = 0; m < NPRE; ++m) accv[m] = (E >= 2) ? a.acc[ro + t + m * Q] : cmk(0.0, 0.0);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) ysv[m] = need_ys ? a.ys[ro + t + m * Q] : cmk(0.0, 0.0);
+      for (int m = 0; m < NPRE; ++m) ysv[m] = need_ys ? a.ys[ro + t + m * Q] : cmk(0.0, 0.0);
+    }
   } else {
 #pragma unroll
     for (int m = 0; m < 8; ++m) y[m] = a.th[ro + t + m * Q];
@@ -534,9 +541,16 @@ __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
     for (int m = 0; m < 8; ++m) {
       ModeIn in;
       in.J = v[m];
-      in.th = thv[m];
-      in.acc = accv[m];
-      in.ys = ysv[m];
+      if (LEAN) {
+        const size_t o = ro + t + m * Q;
+        in.th = a.th[o];
+        in.acc = (E >= 2) ? a.acc[o] : cmk(0.0, 0.0);
+        in.ys = need_ys ? a.ys[o] : cmk(0.0, 0.0);
+      } else {
+        in.th = thv[m];
+        in.acc = accv[m];
+        in.ys = ysv[m];
+      }
       const bool selfc = row0 && t == 0 && (m == 0 || m == 4);
       ModeOut o;
       if (!selfc) {
@@ -1260,6 +1274,38 @@ __global__ void __launch_bounds__(256) phys_diag_kernel(const double* __restrict
   }
 }
 
+// Per-block maxima for the CFL controller: part[2 b] = max sqrt(u^2 + v^2),
+// part[2 b + 1] = max |X| over a complex array; a NaN anywhere makes the
+// block's maximum NaN.
+__device__ __forceinline__ double nan_max(double m, double x) { return (x != x || x > m) ? x : m; }
+
+__global__ void __launch_bounds__(256) cfl_kernel(const double* __restrict__ u,
+                                                  const double* __restrict__ v,
+                                                  const double2* __restrict__ X, long long n,
+                                                  long long nx, double* part) {
+  __shared__ double s0[256], s1[256];
+  const int tid = threadIdx.x;
+  double a = 0.0, b = 0.0;
+  for (long long e = (long long)blockIdx.x * 256 + tid; e < n; e += (long long)gridDim.x * 256)
+    a = nan_max(a, sqrt(u[e] * u[e] + v[e] * v[e]));
+  for (long long e = (long long)blockIdx.x * 256 + tid; e < nx; e += (long long)gridDim.x * 256)
+    b = nan_max(b, hypot(X[e].x, X[e].y));
+  s0[tid] = a;
+  s1[tid] = b;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (tid < h) {
+      s0[tid] = nan_max(s0[tid], s0[tid + h]);
+      s1[tid] = nan_max(s1[tid], s1[tid + h]);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    part[(size_t)blockIdx.x * 2] = s0[0];
+    part[(size_t)blockIdx.x * 2 + 1] = s1[0];
+  }
+}
+
 // Sum (or max, for slots >= nsum) the per-block partials slot by slot, in
 // block order (deterministic).
 __global__ void __launch_bounds__(256) diag_reduce_kernel(const double* part, int nblk, int stride,
@@ -1511,6 +1557,11 @@ hipError_t launch_spec_diag(const cd* X, const double* k1d, const double* edges,
 hipError_t launch_phys_diag(const double* th, const double* u, const double* v, long long n,
                             double* part, hipStream_t s) {
   hipLaunchKernelGGL(phys_diag_kernel, dim3(diag_blocks()), dim3(256), 0, s, th, u, v, n, part);
+  return hipGetLastError();
+}
+hipError_t launch_cfl(const double* u, const double* v, const double2* X, long long n,
+                      long long nx, double* part, hipStream_t s) {
+  hipLaunchKernelGGL(cfl_kernel, dim3(diag_blocks()), dim3(256), 0, s, u, v, X, n, nx, part);
   return hipGetLastError();
 }
 hipError_t launch_diag_reduce(const double* part, int nblk, int stride, int nsum, double* out,

@@ -223,6 +223,13 @@ class Plan:
 
     KERNEL_CLASSES = ("theta_column", "theta_row", "scalar_column", "scalar_row", "other")
 
+    def cfl_extrema(self, X: torch.Tensor):
+        """(max |u|, max |X|) of a full spectrum (gsqg_cfl_extrema); synchronises."""
+        out = (ctypes.c_double * 2)()
+        _lib.check(self.lib.gsqg_cfl_extrema(self.handle, _ptr(X), out, _stream()),
+                   "gsqg_cfl_extrema")
+        return out[0], out[1]
+
     def profile_advance(self, dt: float, n_steps: int, key_mode: int, key: int, step0: int = 0):
         """Advance with a HIP event pair around every kernel; returns
         {class: (total_ms, launches)}."""

@@ -179,7 +179,7 @@ def test_closed_rhs_vs_oracle(pg):
 
 
 @pytest.mark.parametrize("scheme,every,n", [("if", 0, 5), ("explicit", 0, 5), ("if", 1, 5),
-                                            ("if", 4, 13)])
+                                            ("if", 4, 21)])
 def test_closed_steps_vs_oracle(pg, scheme, every, n):
     N, L, dt = 64, 2 * np.pi, 1e-3
     g, plan, net, params, in_scale, out_scale = _closed_setup(pg, N, scheme, every)
