@@ -319,9 +319,14 @@ def bench_batch(args):
     thetas = [torch.from_numpy(ics.kolmogorov_53(N, seed=100 + m, alpha=a)).to(dev)
               for m, (N, a) in enumerate(members)]
     ens = pg.EnsembleSolver(members, nu_p=nu, p=8, scheme="if", device=dev)
-    st = ens.initialize(thetas)
-    st = ens.multistep(st, args.warmup, dt)
+    st0 = ens.initialize(thetas)
+    # default tile shapes, then the autotuner's choice (tuning is untimed)
+    st = ens.multistep(st0, args.warmup, dt)
+    st_d, sec_d = _timed(lambda: ens.multistep(st, steps, dt), steps)
+    tuned = ens.autotune(dt)
+    st = ens.multistep(st0, args.warmup, dt)
     st, sec = _timed(lambda: ens.multistep(st, steps, dt), steps)
+    same = all(bool(torch.equal(a, b)) for a, b in zip(st["theta_hat"], st_d["theta_hat"]))
     finite = all(bool(torch.isfinite(t).all()) for t in st["theta_hat"])
     pts = sum(N * N for N, _ in members)
     value = pts * steps / sec
@@ -348,6 +353,11 @@ def bench_batch(args):
         "config": {"workload": "96 members: 32 each at N=128, 256, 512; alpha cycling "
                                "{0, 0.5, 1, 1.5, 2}; nu_p=1e-4, p=8, dt=1e-3, IF scheme",
                    "members": len(members), "finite": finite},
+        "autotune": {"shapes": {str(N): list(b) for N, (b, _) in tuned.items()},
+                     "ms_per_pair": {str(N): {f"{c},{r}": round(ms, 4) for (c, r), ms in t.items()}
+                                     for N, (_, t) in tuned.items()},
+                     "default_shapes_value": pts * steps / sec_d,
+                     "identical_to_default": same},
         "step_roofline": {"bound": "hbm", "achieved": B * steps / sec / 1e9,
                           "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                           "frac": B * steps / sec / PEAK_HBM, "bytes_per_step": B},

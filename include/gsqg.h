@@ -168,6 +168,25 @@ int gsqg_ring_forcing(gsqg_plan* plan, const void* theta_hat_dev, uint64_t key, 
 int gsqg_profile_advance(gsqg_plan* plan, double dt, int n_steps, int key_mode, uint64_t key,
                          int64_t step0, double* class_ms, int* class_count, void* stream);
 
+/* Tile-shape autotuner (no counterpart in the reference, whose FFTs are
+ * XLA's).  The column and row kernels are compiled in several tile shapes
+ * for n <= 512 (n_variants = 3: the default 128-thread tile, a tile twice as
+ * tall, one half as tall); larger n has one shape.  gsqg_autotune times
+ * gsqg_advance over n_steps steps (graphs on, explicit key 0) for every
+ * (column, row) shape pair, best of `repeats` runs each, keeps the fastest
+ * pair in the plan and returns it in best[2]; ms_table (n_variants^2 host
+ * doubles, or NULL) receives the time of pair (c, r) at c * n_variants + r.
+ * The plan's state (theta, ring side state, scalars) is saved into a
+ * temporary device buffer and restored afterwards; synchronises stream.
+ * gsqg_plan_set_shape / gsqg_plan_shape set and read the pair directly;
+ * captured graphs are dropped.  Every pair gives bit-identical states except
+ * with ring forcing, whose <theta f> partial sums are then added in another
+ * order (differences at the rounding level). */
+int gsqg_autotune(gsqg_plan* plan, double dt, int n_steps, int repeats, int* best,
+                  double* ms_table, void* stream);
+int gsqg_plan_set_shape(gsqg_plan* plan, int col_shape, int row_shape);
+int gsqg_plan_shape(const gsqg_plan* plan, int* col_shape, int* row_shape, int* n_variants);
+
 /* Non-finite flag raised by the last steps (synchronises stream). */
 int gsqg_nonfinite(gsqg_plan* plan, int* flag, int clear, void* stream);
 

@@ -100,6 +100,12 @@ _SIGS = {
                                             ctypes.c_uint64, ctypes.c_int64,
                                             ctypes.POINTER(ctypes.c_double),
                                             ctypes.POINTER(ctypes.c_int), _VP]),
+    "gsqg_autotune": (ctypes.c_int, [_VP, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
+                                     _VP]),
+    "gsqg_plan_set_shape": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int]),
+    "gsqg_plan_shape": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int),
+                                       ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "gsqg_nonfinite": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int), ctypes.c_int, _VP]),
     "gsqg_forcing_stats": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_double), _VP]),

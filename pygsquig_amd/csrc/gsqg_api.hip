@@ -4,6 +4,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -13,11 +14,12 @@
 #include "gsqg_internal.h"
 
 namespace gsqg {
-hipError_t launch_ck(int N, int job, const CKArgs& a, int ny, hipStream_t s,
-                     const LaunchEv* ev = nullptr);
-hipError_t launch_rk(int N, int job, const RKArgs& a, int ny, hipStream_t s,
-                     const LaunchEv* ev = nullptr);
-int ck_num_blocks(int N);
+hipError_t launch_ck(int N, int job, const CKArgs& a, int ny, hipStream_t s, const LaunchEv* ev,
+                     int var);
+hipError_t launch_rk(int N, int job, const RKArgs& a, int ny, hipStream_t s, const LaunchEv* ev,
+                     int var);
+int ck_num_blocks(int N, int var);
+int num_shape_variants(int N);
 hipError_t launch_full_to_half(const double2* X, double2* H, int N, double* norms, double* norm0,
                                hipStream_t s);
 int full_to_half_num_blocks(int N);
@@ -90,7 +92,9 @@ struct gsqg_plan {
   double2* tw = nullptr;
   Ctrl* ctrl = nullptr;
   double* part = nullptr;
-  int part_stride = 0;
+  int part_stride = 0;  // >= the column kernel's block count in every shape variant
+  // tile-shape variants of the column and row kernels (gsqg_autotune)
+  int ck_var = 0, rk_var = 0, n_var = 1;
   int2* ring = nullptr;
   ATab* a_tab = nullptr;
   double2* a_val = nullptr;
@@ -253,7 +257,9 @@ extern "C" int gsqg_plan_create(const gsqg_plan_desc* d, gsqg_plan** out) {
   ALLOC(p->flags, h);
   ALLOC(p->tw, (size_t)2 * 8192);
   ALLOC(p->ctrl, 1);
-  p->part_stride = ck_num_blocks(N);
+  p->n_var = num_shape_variants(N);
+  for (int v = 0; v < p->n_var; ++v)
+    p->part_stride = std::max(p->part_stride, ck_num_blocks(N, v));
   ALLOC(p->part, (size_t)8 * p->part_stride);
   p->n_ring = d->forcing ? d->n_ring : 0;
   ALLOC(p->ring, (size_t)(p->n_ring > 0 ? p->n_ring : 1));
@@ -340,20 +346,20 @@ static hipError_t prof_pair(gsqg_plan* p, int cls, LaunchEv* ev) {
 
 static hipError_t ck(gsqg_plan* p, int job, const CKArgs& a, int ny, hipStream_t st) {
   const int cls = job == CK_THETA ? KC_THETA_COL : job == CK_SCALAR ? KC_SCALAR_COL : KC_OTHER;
-  if (!p->prof) return launch_ck(p->N, job, a, ny, st);
+  if (!p->prof) return launch_ck(p->N, job, a, ny, st, nullptr, p->ck_var);
   LaunchEv ev{};
   hipError_t r = prof_pair(p, cls, &ev);
-  return r != hipSuccess ? r : launch_ck(p->N, job, a, ny, st, &ev);
+  return r != hipSuccess ? r : launch_ck(p->N, job, a, ny, st, &ev, p->ck_var);
 }
 
 static hipError_t rk(gsqg_plan* p, int job, const RKArgs& a, int ny, hipStream_t st) {
   const int cls = (job == RK_THETA || job == RK_THETA_CL) ? KC_THETA_ROW
                   : job == RK_SCALAR                         ? KC_SCALAR_ROW
                                                              : KC_OTHER;
-  if (!p->prof) return launch_rk(p->N, job, a, ny, st);
+  if (!p->prof) return launch_rk(p->N, job, a, ny, st, nullptr, p->rk_var);
   LaunchEv ev{};
   hipError_t r = prof_pair(p, cls, &ev);
-  return r != hipSuccess ? r : launch_rk(p->N, job, a, ny, st, &ev);
+  return r != hipSuccess ? r : launch_rk(p->N, job, a, ny, st, &ev, p->rk_var);
 }
 
 // ----------------------------------------------------------- arg builders
@@ -401,7 +407,7 @@ static RKArgs rk_theta(gsqg_plan* p, int red_slot, int inc_step, bool uv_out) {
   a.ctrl = p->ctrl;
   a.part = p->part;
   a.part_stride = p->part_stride;
-  a.nblk_ck = p->part_stride;
+  a.nblk_ck = ck_num_blocks(p->N, p->ck_var);
   a.red_slot = p->ph.forcing ? red_slot : 0;
   a.inc_step = inc_step;
   a.eps = p->ph.eps;
@@ -880,6 +886,128 @@ extern "C" int gsqg_profile_advance(gsqg_plan* p, double dt, int n_steps, int ke
     }
   }
   return GSQG_OK;
+}
+
+// ------------------------------------------------------------- autotuner
+static int drop_graphs(gsqg_plan* p) {
+  if (p->gexec) HIPCHK(hipGraphExecDestroy(p->gexec));
+  if (p->gexec_k) HIPCHK(hipGraphExecDestroy(p->gexec_k));
+  p->gexec = p->gexec_k = nullptr;
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_plan_set_shape(gsqg_plan* p, int col_shape, int row_shape) {
+  if (!p) return fail(GSQG_ERR_ARG, "null plan");
+  if (col_shape < 0 || col_shape >= p->n_var || row_shape < 0 || row_shape >= p->n_var)
+    return fail(GSQG_ERR_ARG, "shape variant out of range [0, " + std::to_string(p->n_var) +
+                                  ") for N = " + std::to_string(p->N));
+  if (col_shape == p->ck_var && row_shape == p->rk_var) return GSQG_OK;
+  HIPCHK(hipDeviceSynchronize());  // a replayed graph may still use the old kernels
+  int rc = drop_graphs(p);
+  if (rc) return rc;
+  p->ck_var = col_shape;
+  p->rk_var = row_shape;
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_plan_shape(const gsqg_plan* p, int* col_shape, int* row_shape,
+                               int* n_variants) {
+  if (!p) return fail(GSQG_ERR_ARG, "null plan");
+  if (col_shape) *col_shape = p->ck_var;
+  if (row_shape) *row_shape = p->rk_var;
+  if (n_variants) *n_variants = p->n_var;
+  return GSQG_OK;
+}
+
+// Device copies of the evolving state: theta (all members), the ring side
+// state, the scalars and the control block.
+struct StateSave {
+  double2* buf = nullptr;
+  size_t n_th = 0, n_a = 0, n_s = 0;
+};
+
+static int state_copy(gsqg_plan* p, StateSave& sv, bool save, hipStream_t st) {
+  const size_t b = sizeof(double2);
+  double2* th = sv.buf;
+  double2* av = th + sv.n_th;
+  double2* sc = av + sv.n_a;
+  Ctrl* ct = reinterpret_cast<Ctrl*>(sc + sv.n_s);
+  const hipMemcpyKind k = hipMemcpyDeviceToDevice;
+  if (save) {
+    HIPCHK(hipMemcpyAsync(th, p->th, sv.n_th * b, k, st));
+    if (sv.n_a) HIPCHK(hipMemcpyAsync(av, p->a_val, sv.n_a * b, k, st));
+    if (sv.n_s) HIPCHK(hipMemcpyAsync(sc, p->s_th, sv.n_s * b, k, st));
+    HIPCHK(hipMemcpyAsync(ct, p->ctrl, sizeof(Ctrl), k, st));
+  } else {
+    HIPCHK(hipMemcpyAsync(p->th, th, sv.n_th * b, k, st));
+    if (sv.n_a) HIPCHK(hipMemcpyAsync(p->a_val, av, sv.n_a * b, k, st));
+    if (sv.n_s) HIPCHK(hipMemcpyAsync(p->s_th, sc, sv.n_s * b, k, st));
+    HIPCHK(hipMemcpyAsync(p->ctrl, ct, sizeof(Ctrl), k, st));
+  }
+  return GSQG_OK;
+}
+
+// Time every (column, row) shape pair on n_steps graphed steps and keep the
+// fastest.  All pairs run the same arithmetic, so the choice changes speed
+// only.
+static int autotune_body(gsqg_plan* p, double dt, int n_steps, int repeats, int* best,
+                         double* ms_table, hipStream_t st, StateSave& sv, hipEvent_t e0,
+                         hipEvent_t e1) {
+  int rc;
+  const int nv = p->n_var;
+  double best_ms = 1e300;
+  int bc = p->ck_var, br = p->rk_var;
+  for (int c = 0; c < nv; ++c) {
+    for (int r = 0; r < nv; ++r) {
+      if ((rc = gsqg_plan_set_shape(p, c, r))) return rc;
+      double t = 1e300;
+      for (int k = 0; k <= repeats; ++k) {  // k = 0 captures the graphs
+        if ((rc = state_copy(p, sv, false, st))) return rc;
+        HIPCHK(hipEventRecord(e0, st));
+        if ((rc = gsqg_advance(p, dt, n_steps, GSQG_KEY_EXPLICIT, 0, 0, 1, st))) return rc;
+        HIPCHK(hipEventRecord(e1, st));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (k > 0) t = std::min(t, (double)ms);
+      }
+      if (ms_table) ms_table[c * nv + r] = t;
+      if (t < best_ms) {
+        best_ms = t;
+        bc = c;
+        br = r;
+      }
+    }
+  }
+  if ((rc = state_copy(p, sv, false, st))) return rc;
+  if ((rc = gsqg_plan_set_shape(p, bc, br))) return rc;
+  best[0] = bc;
+  best[1] = br;
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_autotune(gsqg_plan* p, double dt, int n_steps, int repeats, int* best,
+                             double* ms_table, void* stream) {
+  if (!p || !best) return fail(GSQG_ERR_ARG, "null argument");
+  if (n_steps < 1 || repeats < 1) return fail(GSQG_ERR_ARG, "n_steps and repeats must be >= 1");
+  hipStream_t st = (hipStream_t)stream;
+  StateSave sv;
+  sv.n_th = p->hsz * (size_t)p->n_mem;
+  sv.n_a = (size_t)p->n_ring;
+  sv.n_s = p->hsz * (size_t)p->n_scalars;
+  const size_t nb = (sv.n_th + sv.n_a + sv.n_s) * sizeof(double2) + sizeof(Ctrl);
+  HIPCHK(hipMalloc((void**)&sv.buf, nb));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = GSQG_OK;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    rc = fail(GSQG_ERR_HIP, "hipEventCreate failed");
+  if (!rc) rc = state_copy(p, sv, true, st);
+  if (!rc) rc = autotune_body(p, dt, n_steps, repeats, best, ms_table, st, sv, e0, e1);
+  (void)hipStreamSynchronize(st);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(sv.buf);
+  return rc;
 }
 
 extern "C" int gsqg_nonfinite(gsqg_plan* p, int* flag, int clear, void* stream) {

@@ -25,10 +25,15 @@
 namespace gsqg {
 
 // ------------------------------------------------------------------ shapes
-template <int N>
+// Shape variant V, chosen per plan by the autotuner (gsqg_autotune): 0 is
+// the default tile of 128 threads; 1 doubles and 2 halves the rows (column
+// kernel) or x-rows (row kernel) of a block.  Variants 1 and 2 are compiled
+// for N <= 512.
+template <int N, int V = 0>
 struct CKShape {
   static constexpr int Q = N / 8;  // lanes per row
-  static constexpr int R0 = (128 / Q) > 1 ? (128 / Q) : 1;
+  static constexpr int R00 = (128 / Q) > 1 ? (128 / Q) : 1;
+  static constexpr int R0 = V == 1 ? 2 * R00 : (V == 2 && R00 > 1 ? R00 / 2 : R00);
   static constexpr int RPW = R0 < (N / 2) ? R0 : (N / 2);  // rows per block
   static constexpr int NT = RPW * Q;
   static constexpr int PADL = Padded<N>::value;
@@ -39,11 +44,12 @@ struct CKShape {
   static constexpr int FP = REGION * 16 * 2 <= 80 * 1024 ? 2 : 1;
 };
 
-template <int N>
+template <int N, int V = 0>
 struct RKShape {
   static constexpr int M = N / 2;
   static constexpr int Q = M / 8;  // lanes per x-row
-  static constexpr int W0 = (128 / Q) > 1 ? (128 / Q) : 1;
+  static constexpr int W00 = (128 / Q) > 1 ? (128 / Q) : 1;
+  static constexpr int W0 = V == 1 ? 2 * W00 : (V == 2 && W00 > 1 ? W00 / 2 : W00);
   static constexpr int WX = W0 < N ? W0 : N;  // x-rows per block
   static constexpr int NT = WX * Q;
   static constexpr int PADM = Padded<M>::value;
@@ -382,9 +388,9 @@ __device__ __forceinline__ ModeOut row0_selfc_epilogue(int E, int r, const ModeI
   return o;
 }
 
-template <int N, int JOB, int E, bool FORCED>
-__global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
-  using S = CKShape<N>;
+template <int N, int JOB, int E, bool FORCED, int V>
+__global__ void __launch_bounds__((CKShape<N, V>::NT)) ck_kernel(CKArgs a) {
+  using S = CKShape<N, V>;
   constexpr int Q = S::Q;
   constexpr int PADL = S::PADL;
   constexpr int NT = S::NT;
@@ -735,9 +741,9 @@ __global__ void __launch_bounds__(CKShape<N>::NT) ck_kernel(CKArgs a) {
 }
 
 // --------------------------------------------------------------- row kernel
-template <int N>
+template <int N, int V>
 __device__ __forceinline__ int rk_tile(int b) {
-  constexpr int T = N / RKShape<N>::WX;
+  constexpr int T = N / RKShape<N, V>::WX;
   if (T % 8 != 0) return b;
   // Blocks are dispatched round-robin over the 8 XCDs: give each XCD a
   // contiguous run of x-tiles so that tiles sharing 128-byte lines of the
@@ -749,10 +755,10 @@ __device__ __forceinline__ int rk_tile(int b) {
 // coalesced along x) into registers.  Staged as native 2-vectors: plain
 // values that stay in registers (an array of the aggregate type is copied
 // through scratch).
-template <int N, int G>
+template <int N, int V, int G>
 __device__ __forceinline__ void rk_issue(const cd* const (&src)[G], int x0, int tid,
                                          vd2 (&tmp)[G * 8]) {
-  using S = RKShape<N>;
+  using S = RKShape<N, V>;
   constexpr int WX = S::WX, NT = S::NT;
 #pragma unroll
   for (int e = 0; e < G * 8; ++e) {
@@ -778,10 +784,10 @@ struct PreTw {
 // gets Z'(k) = (X(k) + conj X(M-k)) + i e^{+2 pi i k/N} (X(k) - conj X(M-k)),
 // with pre(k) = e^{-2 pi i k/N}.  Ends without a barrier: the next LDS write
 // (inside the transform) is preceded by one.
-template <int N, int G, class PT>
+template <int N, int V, int G, class PT>
 __device__ __forceinline__ void rk_commit(const vd2 (&tmp)[G * 8], cd* lds, int tid, int b, int t,
                                           const PT& pre, cd (&v)[G][8]) {
-  using S = RKShape<N>;
+  using S = RKShape<N, V>;
   constexpr int M = S::M, WX = S::WX, NT = S::NT, PADM = S::PADM, Q = S::Q, RG = S::REGION;
   __syncthreads();
 #pragma unroll
@@ -811,10 +817,10 @@ __device__ __forceinline__ void rk_commit(const vd2 (&tmp)[G * 8], cd* lds, int 
 
 // Forward half-length FFT result in v -> post-twist -> store X(k), packed
 // slot 0 = X(0) + i X(M), at dst[k*N + x0 + b] (coalesced along x).
-template <int N, class PT>
+template <int N, int V, class PT>
 __device__ __forceinline__ void rk_store_r2c(cd (&v)[8], cd* dst, int x0, cd* lds, int tid, int b,
                                              int t, const PT& pre) {
-  using S = RKShape<N>;
+  using S = RKShape<N, V>;
   constexpr int M = S::M, WX = S::WX, NT = S::NT, PADM = S::PADM, Q = S::Q;
   cd* buf = lds + b * PADM;
   __syncthreads();
@@ -840,9 +846,9 @@ __device__ __forceinline__ void rk_store_r2c(cd (&v)[8], cd* dst, int x0, cd* ld
   }
 }
 
-template <int N, int JOB>
-__global__ void __launch_bounds__(RKShape<N>::NT) rk_kernel(RKArgs a) {
-  using S = RKShape<N>;
+template <int N, int JOB, int V>
+__global__ void __launch_bounds__((RKShape<N, V>::NT)) rk_kernel(RKArgs a) {
+  using S = RKShape<N, V>;
   constexpr int M = S::M, Q = S::Q, WX = S::WX, NT = S::NT, PADM = S::PADM, RG = S::REGION;
   constexpr int NF = RKFields<JOB>::value;
   constexpr int G = S::FP < NF ? S::FP : NF;  // fields per lockstep group
@@ -856,7 +862,7 @@ __global__ void __launch_bounds__(RKShape<N>::NT) rk_kernel(RKArgs a) {
   const int tid = threadIdx.x;
   const int b = tid / Q;
   const int t = tid - b * Q;
-  const int x0 = rk_tile<N>(blockIdx.x) * WX;
+  const int x0 = rk_tile<N, V>(blockIdx.x) * WX;
   const int x = x0 + b;
   const size_t fo = (size_t)blockIdx.y * (size_t)a.field_stride;
   const cd* __restrict__ tw = a.tw;
@@ -901,7 +907,7 @@ __global__ void __launch_bounds__(RKShape<N>::NT) rk_kernel(RKArgs a) {
       const int q = THETA ? ((g & 1) ? 2 + (g >> 1) : (g >> 1)) : g;
       src[g] = a.F[q] + fo;
     }
-    rk_issue<N, G>(src, x0, tid, tmp);
+    rk_issue<N, V, G>(src, x0, tid, tmp);
   }
   // closure flux of this x-line as packed real pairs (y = 2k, 2k + 1)
   typedef float vf2 __attribute__((vector_size(8)));
@@ -931,7 +937,7 @@ __global__ void __launch_bounds__(RKShape<N>::NT) rk_kernel(RKArgs a) {
       fft_core<M, -1, 1>(*reinterpret_cast<cd(*)[1][8]>(&v), buf, 0, t, twS);
     else
       fft_core<M, -1, 1>(*reinterpret_cast<cd(*)[1][8]>(&v), buf, 0, t, twG);
-    rk_store_r2c<N>(v, a.Jout + fo, x0, lds, tid, b, t, pre);
+    rk_store_r2c<N, V>(v, a.Jout + fo, x0, lds, tid, b, t, pre);
     return;
   }
 
@@ -942,7 +948,7 @@ __global__ void __launch_bounds__(RKShape<N>::NT) rk_kernel(RKArgs a) {
 #pragma unroll
   for (int g0 = 0; g0 < NF; g0 += G) {
     cd w[G][8];
-    rk_commit<N, G>(tmp, lds, tid, b, t, pre, w);
+    rk_commit<N, V, G>(tmp, lds, tid, b, t, pre, w);
     if (TWL)
       fft_core<M, +1, G>(w, buf, RG, t, twS);
     else
@@ -955,7 +961,7 @@ __global__ void __launch_bounds__(RKShape<N>::NT) rk_kernel(RKArgs a) {
         const int q = THETA ? ((s & 1) ? 2 + (s >> 1) : (s >> 1)) : s;
         src[g] = a.F[q] + fo;
       }
-      rk_issue<N, G>(src, x0, tid, tmp);
+      rk_issue<N, V, G>(src, x0, tid, tmp);
     }
 #pragma unroll
     for (int g = 0; g < G; ++g) {
@@ -1009,7 +1015,7 @@ __global__ void __launch_bounds__(RKShape<N>::NT) rk_kernel(RKArgs a) {
     fft_core<M, -1, 1>(*reinterpret_cast<cd(*)[1][8]>(&v), buf, 0, t, twS);
   else
     fft_core<M, -1, 1>(*reinterpret_cast<cd(*)[1][8]>(&v), buf, 0, t, twG);
-  rk_store_r2c<N>(v, a.Jout + fo, x0, lds, tid, b, t, pre);
+  rk_store_r2c<N, V>(v, a.Jout + fo, x0, lds, tid, b, t, pre);
 }
 
 #if defined(GSQG_INSTANTIATE_COMMON)
@@ -1403,111 +1409,150 @@ __global__ void set_ctrl_kernel(Ctrl* c, int64_t step, uint64_t key, uint64_t se
 // ------------------------------------------------------------- launchers
 // The build compiles this file once per grid size (GSQG_INSTANTIATE_N) and
 // once for the size-independent parts (GSQG_INSTANTIATE_COMMON).
-template <int N, int JOB, int E, bool FORCED>
+template <int N, int JOB, int E, bool FORCED, int V>
 static hipError_t launch_ck_n(const CKArgs& a, int nfields_y, hipStream_t s, const LaunchEv* ev) {
-  using S = CKShape<N>;
+  using S = CKShape<N, V>;
   dim3 grid((N / 2) / S::RPW, nfields_y);
   if (ev)
-    hipExtLaunchKernelGGL((ck_kernel<N, JOB, E, FORCED>), grid, dim3(S::NT), 0, s, ev->start,
+    hipExtLaunchKernelGGL((ck_kernel<N, JOB, E, FORCED, V>), grid, dim3(S::NT), 0, s, ev->start,
                           ev->stop, 0, a);
   else
-    hipLaunchKernelGGL((ck_kernel<N, JOB, E, FORCED>), grid, dim3(S::NT), 0, s, a);
+    hipLaunchKernelGGL((ck_kernel<N, JOB, E, FORCED, V>), grid, dim3(S::NT), 0, s, a);
   return hipGetLastError();
 }
 
-template <int N, int JOB, bool FORCED>
+template <int N, int JOB, bool FORCED, int V>
 static hipError_t launch_ck_e(const CKArgs& a, int ny, hipStream_t s, const LaunchEv* ev) {
   switch (a.e) {
-    case 0: return launch_ck_n<N, JOB, 0, FORCED>(a, ny, s, ev);
-    case 1: return launch_ck_n<N, JOB, 1, FORCED>(a, ny, s, ev);
-    case 2: return launch_ck_n<N, JOB, 2, FORCED>(a, ny, s, ev);
-    case 3: return launch_ck_n<N, JOB, 3, FORCED>(a, ny, s, ev);
-    case 4: return launch_ck_n<N, JOB, 4, FORCED>(a, ny, s, ev);
+    case 0: return launch_ck_n<N, JOB, 0, FORCED, V>(a, ny, s, ev);
+    case 1: return launch_ck_n<N, JOB, 1, FORCED, V>(a, ny, s, ev);
+    case 2: return launch_ck_n<N, JOB, 2, FORCED, V>(a, ny, s, ev);
+    case 3: return launch_ck_n<N, JOB, 3, FORCED, V>(a, ny, s, ev);
+    case 4: return launch_ck_n<N, JOB, 4, FORCED, V>(a, ny, s, ev);
   }
   return hipErrorInvalidValue;
 }
 
-template <int N, int JOB>
+template <int N, int JOB, int V>
 static hipError_t launch_rk_n(const RKArgs& a, int nfields_y, hipStream_t s, const LaunchEv* ev) {
-  using S = RKShape<N>;
+  using S = RKShape<N, V>;
   dim3 grid(N / S::WX, nfields_y);
   if (ev)
-    hipExtLaunchKernelGGL((rk_kernel<N, JOB>), grid, dim3(S::NT), 0, s, ev->start, ev->stop, 0, a);
+    hipExtLaunchKernelGGL((rk_kernel<N, JOB, V>), grid, dim3(S::NT), 0, s, ev->start, ev->stop, 0,
+                          a);
   else
-    hipLaunchKernelGGL((rk_kernel<N, JOB>), grid, dim3(S::NT), 0, s, a);
+    hipLaunchKernelGGL((rk_kernel<N, JOB, V>), grid, dim3(S::NT), 0, s, a);
   return hipGetLastError();
 }
 
-template <int N>
-hipError_t ck_by_job(int job, const CKArgs& a, int ny, hipStream_t s, const LaunchEv* ev) {
+template <int N, int V>
+static hipError_t ck_by_job_v(int job, const CKArgs& a, int ny, hipStream_t s, const LaunchEv* ev) {
   switch (job) {
     case CK_THETA:
-      return a.ph.forcing ? launch_ck_e<N, CK_THETA, true>(a, ny, s, ev)
-                          : launch_ck_e<N, CK_THETA, false>(a, ny, s, ev);
-    case CK_SCALAR: return launch_ck_e<N, CK_SCALAR, false>(a, ny, s, ev);
-    case CK_VEL: return launch_ck_n<N, CK_VEL, 0, false>(a, ny, s, ev);
-    case CK_FWD: return launch_ck_n<N, CK_FWD, 0, false>(a, ny, s, ev);
-    case CK_INV: return launch_ck_n<N, CK_INV, 0, false>(a, ny, s, ev);
+      return a.ph.forcing ? launch_ck_e<N, CK_THETA, true, V>(a, ny, s, ev)
+                          : launch_ck_e<N, CK_THETA, false, V>(a, ny, s, ev);
+    case CK_SCALAR: return launch_ck_e<N, CK_SCALAR, false, V>(a, ny, s, ev);
+    case CK_VEL: return launch_ck_n<N, CK_VEL, 0, false, V>(a, ny, s, ev);
+    case CK_FWD: return launch_ck_n<N, CK_FWD, 0, false, V>(a, ny, s, ev);
+    case CK_INV: return launch_ck_n<N, CK_INV, 0, false, V>(a, ny, s, ev);
   }
   return hipErrorInvalidValue;
 }
 
-template <int N>
-hipError_t rk_by_job(int job, const RKArgs& a, int ny, hipStream_t s, const LaunchEv* ev) {
+template <int N, int V>
+static hipError_t rk_by_job_v(int job, const RKArgs& a, int ny, hipStream_t s, const LaunchEv* ev) {
   switch (job) {
-    case RK_THETA: return launch_rk_n<N, RK_THETA>(a, ny, s, ev);
-    case RK_SCALAR: return launch_rk_n<N, RK_SCALAR>(a, ny, s, ev);
-    case RK_VEL: return launch_rk_n<N, RK_VEL>(a, ny, s, ev);
-    case RK_R2C: return launch_rk_n<N, RK_R2C>(a, ny, s, ev);
-    case RK_C2R: return launch_rk_n<N, RK_C2R>(a, ny, s, ev);
-    case RK_THETA_CL: return launch_rk_n<N, RK_THETA_CL>(a, ny, s, ev);
+    case RK_THETA: return launch_rk_n<N, RK_THETA, V>(a, ny, s, ev);
+    case RK_SCALAR: return launch_rk_n<N, RK_SCALAR, V>(a, ny, s, ev);
+    case RK_VEL: return launch_rk_n<N, RK_VEL, V>(a, ny, s, ev);
+    case RK_R2C: return launch_rk_n<N, RK_R2C, V>(a, ny, s, ev);
+    case RK_C2R: return launch_rk_n<N, RK_C2R, V>(a, ny, s, ev);
+    case RK_THETA_CL: return launch_rk_n<N, RK_THETA_CL, V>(a, ny, s, ev);
   }
   return hipErrorInvalidValue;
 }
 
+// Number of shape variants compiled for grid size N.
 template <int N>
-int ck_blocks_n() { return (N / 2) / CKShape<N>::RPW; }
+constexpr int n_variants() { return N <= 512 ? 3 : 1; }
+
+template <int N>
+hipError_t ck_by_job(int job, const CKArgs& a, int ny, hipStream_t s, const LaunchEv* ev, int var) {
+  if constexpr (n_variants<N>() > 1) {
+    if (var == 1) return ck_by_job_v<N, 1>(job, a, ny, s, ev);
+    if (var == 2) return ck_by_job_v<N, 2>(job, a, ny, s, ev);
+  }
+  return ck_by_job_v<N, 0>(job, a, ny, s, ev);
+}
+
+template <int N>
+hipError_t rk_by_job(int job, const RKArgs& a, int ny, hipStream_t s, const LaunchEv* ev, int var) {
+  if constexpr (n_variants<N>() > 1) {
+    if (var == 1) return rk_by_job_v<N, 1>(job, a, ny, s, ev);
+    if (var == 2) return rk_by_job_v<N, 2>(job, a, ny, s, ev);
+  }
+  return rk_by_job_v<N, 0>(job, a, ny, s, ev);
+}
+
+template <int N>
+int ck_blocks_n(int var) {
+  if constexpr (n_variants<N>() > 1) {
+    if (var == 1) return (N / 2) / CKShape<N, 1>::RPW;
+    if (var == 2) return (N / 2) / CKShape<N, 2>::RPW;
+  }
+  return (N / 2) / CKShape<N, 0>::RPW;
+}
 
 #define GSQG_FOR_EACH_N(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) X(8192)
 
 #if defined(GSQG_INSTANTIATE_N)
 template hipError_t ck_by_job<GSQG_INSTANTIATE_N>(int, const CKArgs&, int, hipStream_t,
-                                                  const LaunchEv*);
+                                                  const LaunchEv*, int);
 template hipError_t rk_by_job<GSQG_INSTANTIATE_N>(int, const RKArgs&, int, hipStream_t,
-                                                  const LaunchEv*);
+                                                  const LaunchEv*, int);
 #endif
 
 #if defined(GSQG_INSTANTIATE_COMMON)
 #define GSQG_EXTERN(N_)                                                              \
   extern template hipError_t ck_by_job<N_>(int, const CKArgs&, int, hipStream_t, \
-                                           const LaunchEv*);                     \
+                                           const LaunchEv*, int);                \
   extern template hipError_t rk_by_job<N_>(int, const RKArgs&, int, hipStream_t, \
-                                           const LaunchEv*);
+                                           const LaunchEv*, int);
 GSQG_FOR_EACH_N(GSQG_EXTERN)
 #undef GSQG_EXTERN
 
 #define GSQG_CASE(N_, CALL) \
   case N_: return CALL;
 
-hipError_t launch_ck(int N, int job, const CKArgs& a, int ny, hipStream_t s, const LaunchEv* ev) {
+hipError_t launch_ck(int N, int job, const CKArgs& a, int ny, hipStream_t s, const LaunchEv* ev,
+                     int var) {
   switch (N) {
-#define X(N_) GSQG_CASE(N_, ck_by_job<N_>(job, a, ny, s, ev))
+#define X(N_) GSQG_CASE(N_, ck_by_job<N_>(job, a, ny, s, ev, var))
     GSQG_FOR_EACH_N(X)
 #undef X
   }
   return hipErrorInvalidValue;
 }
-hipError_t launch_rk(int N, int job, const RKArgs& a, int ny, hipStream_t s, const LaunchEv* ev) {
+hipError_t launch_rk(int N, int job, const RKArgs& a, int ny, hipStream_t s, const LaunchEv* ev,
+                     int var) {
   switch (N) {
-#define X(N_) GSQG_CASE(N_, rk_by_job<N_>(job, a, ny, s, ev))
+#define X(N_) GSQG_CASE(N_, rk_by_job<N_>(job, a, ny, s, ev, var))
     GSQG_FOR_EACH_N(X)
 #undef X
   }
   return hipErrorInvalidValue;
 }
-int ck_num_blocks(int N) {
+int ck_num_blocks(int N, int var) {
   switch (N) {
-#define X(N_) GSQG_CASE(N_, ck_blocks_n<N_>())
+#define X(N_) GSQG_CASE(N_, ck_blocks_n<N_>(var))
+    GSQG_FOR_EACH_N(X)
+#undef X
+  }
+  return 0;
+}
+int num_shape_variants(int N) {
+  switch (N) {
+#define X(N_) GSQG_CASE(N_, n_variants<N_>())
     GSQG_FOR_EACH_N(X)
 #undef X
   }

@@ -100,6 +100,26 @@ class EnsembleSolver:
     def step(self, state: dict, dt: float) -> dict:
         return self.multistep(state, 1, dt)
 
+    def autotune(self, dt: float, n_steps: int = 32, repeats: int = 3) -> dict:
+        """Pick each group's column/row tile shapes by timing every pair on
+        the group's plan (``Plan.autotune``; each group alone, on its own
+        stream).  Returns ``{N: ((col, row), {(col, row): ms})}``; results of
+        later steps are unchanged bit for bit, only their speed."""
+        out = {}
+        cur = torch.cuda.current_stream(self.device)
+        for N, plan in self.plans.items():
+            s = self.streams[N]
+            s.wait_stream(cur)
+            with torch.cuda.stream(s):
+                out[N] = plan.autotune(dt, n_steps, repeats)
+            cur.wait_stream(s)
+        return out
+
+    def set_shapes(self, shapes: dict) -> None:
+        """Set ``{N: (col, row)}`` tile shapes directly (e.g. a saved tuning)."""
+        for N, (c, r) in shapes.items():
+            self.plans[N].set_shape(c, r)
+
 
 def _fft2(t: torch.Tensor, grid):
     from .grid import fft2

@@ -241,6 +241,30 @@ class Plan:
                    "gsqg_profile_advance")
         return {c: (ms[i], cnt[i]) for i, c in enumerate(self.KERNEL_CLASSES)}
 
+    # ------------------------------------------------------------ autotuner
+    def shape(self):
+        """(column shape, row shape, number of shape variants) of this plan."""
+        c, r, n = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+        _lib.check(self.lib.gsqg_plan_shape(self.handle, ctypes.byref(c), ctypes.byref(r),
+                                            ctypes.byref(n)), "gsqg_plan_shape")
+        return c.value, r.value, n.value
+
+    def set_shape(self, col: int, row: int) -> None:
+        _lib.check(self.lib.gsqg_plan_set_shape(self.handle, int(col), int(row)),
+                   "gsqg_plan_set_shape")
+
+    def autotune(self, dt: float, n_steps: int = 32, repeats: int = 3):
+        """Time every (column, row) tile-shape pair on ``n_steps`` graphed
+        steps (gsqg_autotune), keep the fastest; the state is restored.
+        Returns ((col, row), {(col, row): ms})."""
+        nv = self.shape()[2]
+        best = (ctypes.c_int * 2)()
+        table = (ctypes.c_double * (nv * nv))()
+        _lib.check(self.lib.gsqg_autotune(self.handle, float(dt), int(n_steps), int(repeats), best,
+                                          table, _stream()), "gsqg_autotune")
+        times = {(c, r): table[c * nv + r] for c in range(nv) for r in range(nv)}
+        return (best[0], best[1]), times
+
     def nonfinite(self, clear: bool = True) -> bool:
         flag = ctypes.c_int(0)
         _lib.check(self.lib.gsqg_nonfinite(self.handle, ctypes.byref(flag), 1 if clear else 0,
