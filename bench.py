@@ -88,6 +88,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default tile shapes (forced configuration)")
     ap.add_argument("--copy-only", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates PMC byte counters)")
     ap.add_argument("--config", default="forced",
@@ -156,6 +158,9 @@ def main():
     init = {f"s{i}": torch.from_numpy(ics.gaussian_blob(N, c, L / 16)).to(dev)
             for i, c in enumerate(ics.SCALAR_CENTRES[:n_s])}
     state = solver.initialize(theta0, scalar_init=init)
+    tuned = None
+    if not args.no_autotune:  # tile shapes picked by timing every pair (untimed here)
+        tuned = solver.autotune(dt, forcing=forcing, damping=damping)
     state = solver.multistep(state, args.warmup, dt, forcing=forcing, damping=damping)
     torch.cuda.synchronize()
 
@@ -230,6 +235,9 @@ def main():
                      "traffic": read_traffic(dom), "bytes_per_launch": lb[dom],
                      "avg_launch_us": avg_s * 1e6},
         "kernels": kernel_table,
+        "autotune": None if tuned is None else {
+            "shapes": list(tuned[0]),
+            "ms_per_pair": {f"{c},{r}": round(ms, 4) for (c, r), ms in tuned[1].items()}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
@@ -284,6 +292,9 @@ def bench_single(args):
         workload = ("large forced gSQG N=8192, alpha=0.5, ring kf=100, 4 passive scalars, "
                     "dt=1e-4")
     warm = min(args.warmup, 5) if args.config == "large" else args.warmup
+    tuned = None
+    if args.config == "plain" and not args.no_autotune:  # N=8192 has one tile shape
+        tuned = solver.autotune(dt)
     st = run(st, warm)
     st, sec = _timed(lambda: run(st, steps), steps)
     finite = bool(torch.isfinite(st["theta_hat"]).all())
@@ -298,6 +309,9 @@ def bench_single(args):
                    "steps_per_s": sps, "finite": finite},
         "step_roofline": {"bound": "hbm", "achieved": B * sps / 1e9, "peak": PEAK_HBM / 1e9,
                           "unit": "GB/s", "frac": B * sps / PEAK_HBM, "bytes_per_step": B},
+        "autotune": None if tuned is None else {
+            "shapes": list(tuned[0]),
+            "ms_per_pair": {f"{c},{r}": round(ms, 4) for (c, r), ms in tuned[1].items()}},
         "cpu_baseline": None,
         "note": "cpu_baseline is reported for the default configuration only"}))
 
@@ -528,6 +542,13 @@ def bench_closure(args):
             acc += shell(s["theta_hat"])
         return acc / n_samp, sec, bool(torch.isfinite(torch.view_as_real(s["theta_hat"])).all())
 
+    tuned = {}
+    if not args.no_autotune:  # tile shapes of the coarse plans (N=4096 has one shape)
+        tuned["unclosed"] = list(plain.autotune(dt_c, forcing=forcing_c, damping=damping)[0])
+        for m in every_list:
+            tuned[f"closed every={m}"] = list(
+                closed_by[m].autotune(dt_c, forcing=forcing_c, damping=damping)[0])
+        log(f"autotuned coarse tile shapes {tuned}")
     runs = {}
     for dtc in dt_list:
         unclosed = run_coarse(plain, dtc)
@@ -589,6 +610,7 @@ def bench_closure(args):
             for sd, E in spread.items()},
         "wall_clock_s": {"dns_eval": dns_eval_sec, "closed": sec_cl, "unclosed": sec_pl,
                          "training": train_sec},
+        "autotune": tuned,
         "training": {"loss_first": hist[0][1], "loss_last": hist[-1][1],
                      "tflops": train_tflops, "mfma_frac": train_tflops / 2500.0,
                      "forward_ms": inf_ms,

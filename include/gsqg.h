@@ -170,8 +170,9 @@ int gsqg_profile_advance(gsqg_plan* plan, double dt, int n_steps, int key_mode, 
 
 /* Tile-shape autotuner (no counterpart in the reference, whose FFTs are
  * XLA's).  The column and row kernels are compiled in several tile shapes
- * for n <= 512 (n_variants = 3: the default 128-thread tile, a tile twice as
- * tall, one half as tall); larger n has one shape.  gsqg_autotune times
+ * (n_variants = 3 for n <= 512: the default 128-thread tile, a tile twice
+ * as tall, one half as tall; 2 for n = 1024: the default and the taller
+ * tile; 1 above).  gsqg_autotune times
  * gsqg_advance over n_steps steps (graphs on, explicit key 0) for every
  * (column, row) shape pair, best of `repeats` runs each, keeps the fastest
  * pair in the plan and returns it in best[2]; ms_table (n_variants^2 host

@@ -27,8 +27,8 @@ namespace gsqg {
 // ------------------------------------------------------------------ shapes
 // Shape variant V, chosen per plan by the autotuner (gsqg_autotune): 0 is
 // the default tile of 128 threads; 1 doubles and 2 halves the rows (column
-// kernel) or x-rows (row kernel) of a block.  Variants 1 and 2 are compiled
-// for N <= 512.
+// kernel) or x-rows (row kernel) of a block.  Variant 1 is compiled for
+// N <= 1024, variant 2 for N <= 512.
 template <int N, int V = 0>
 struct CKShape {
   static constexpr int Q = N / 8;  // lanes per row
@@ -1472,9 +1472,10 @@ static hipError_t rk_by_job_v(int job, const RKArgs& a, int ny, hipStream_t s, c
   return hipErrorInvalidValue;
 }
 
-// Number of shape variants compiled for grid size N.
+// Number of shape variants compiled for grid size N (at N = 1024 the
+// default column block is one row, so there is no smaller variant).
 template <int N>
-constexpr int n_variants() { return N <= 512 ? 3 : 1; }
+constexpr int n_variants() { return N <= 512 ? 3 : (N == 1024 ? 2 : 1); }
 
 template <int N>
 hipError_t ck_by_job(int job, const CKArgs& a, int ny, hipStream_t s, const LaunchEv* ev, int var) {

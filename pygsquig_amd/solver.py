@@ -96,6 +96,14 @@ class gSQGSolver:
             self._plans[key] = plan
         return plan
 
+    def autotune(self, dt: float, forcing=None, damping=None, n_steps: int = 32,
+                 repeats: int = 3):
+        """Pick the column/row tile shapes of the plan that ``multistep`` with
+        these forcing and damping arguments runs (``Plan.autotune``: every
+        shape pair timed on ``n_steps`` graphed steps, best of ``repeats``;
+        the plan's state is restored).  Returns ((col, row), {(col, row): ms})."""
+        return self._plan(forcing, damping).autotune(dt, n_steps, repeats)
+
     # ------------------------------------------------------------- state
     def initialize(self, theta0=None, seed: Optional[int] = None) -> State:
         """core/solver.py:96-124.  The seed path draws a normal field from
