@@ -180,7 +180,7 @@ int gsqg_profile_advance(gsqg_plan* plan, double dt, int n_steps, int key_mode, 
  * The plan's state (theta, ring side state, scalars) is saved into a
  * temporary device buffer and restored afterwards; synchronises stream.
  * gsqg_plan_set_shape / gsqg_plan_shape set and read the pair directly;
- * captured graphs are dropped.  Every pair gives bit-identical states except
+ * changing the pair synchronises the device and drops captured graphs.  Every pair gives bit-identical states except
  * with ring forcing, whose <theta f> partial sums are then added in another
  * order (differences at the rounding level). */
 int gsqg_autotune(gsqg_plan* plan, double dt, int n_steps, int repeats, int* best,
