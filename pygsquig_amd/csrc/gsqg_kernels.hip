@@ -1479,28 +1479,28 @@ constexpr int n_variants() { return N <= 512 ? 3 : (N == 1024 ? 2 : 1); }
 
 template <int N>
 hipError_t ck_by_job(int job, const CKArgs& a, int ny, hipStream_t s, const LaunchEv* ev, int var) {
-  if constexpr (n_variants<N>() > 1) {
+  if constexpr (n_variants<N>() > 1)
     if (var == 1) return ck_by_job_v<N, 1>(job, a, ny, s, ev);
+  if constexpr (n_variants<N>() > 2)
     if (var == 2) return ck_by_job_v<N, 2>(job, a, ny, s, ev);
-  }
   return ck_by_job_v<N, 0>(job, a, ny, s, ev);
 }
 
 template <int N>
 hipError_t rk_by_job(int job, const RKArgs& a, int ny, hipStream_t s, const LaunchEv* ev, int var) {
-  if constexpr (n_variants<N>() > 1) {
+  if constexpr (n_variants<N>() > 1)
     if (var == 1) return rk_by_job_v<N, 1>(job, a, ny, s, ev);
+  if constexpr (n_variants<N>() > 2)
     if (var == 2) return rk_by_job_v<N, 2>(job, a, ny, s, ev);
-  }
   return rk_by_job_v<N, 0>(job, a, ny, s, ev);
 }
 
 template <int N>
 int ck_blocks_n(int var) {
-  if constexpr (n_variants<N>() > 1) {
+  if constexpr (n_variants<N>() > 1)
     if (var == 1) return (N / 2) / CKShape<N, 1>::RPW;
+  if constexpr (n_variants<N>() > 2)
     if (var == 2) return (N / 2) / CKShape<N, 2>::RPW;
-  }
   return (N / 2) / CKShape<N, 0>::RPW;
 }
 
