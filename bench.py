@@ -90,6 +90,9 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default tile shapes (forced configuration)")
+    ap.add_argument("--shapes", default=None, metavar="COL,ROW",
+                    help="set these column/row tile shapes instead of autotuning (forced "
+                         "configuration; profiles reuse the shapes a timed run picked)")
     ap.add_argument("--copy-only", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates PMC byte counters)")
     ap.add_argument("--config", default="forced",
@@ -159,7 +162,9 @@ def main():
             for i, c in enumerate(ics.SCALAR_CENTRES[:n_s])}
     state = solver.initialize(theta0, scalar_init=init)
     tuned = None
-    if not args.no_autotune:  # tile shapes picked by timing every pair (untimed here)
+    if args.shapes:
+        solver._plan(forcing, damping).set_shape(*(int(x) for x in args.shapes.split(",")))
+    elif not args.no_autotune:  # tile shapes picked by timing every pair (untimed here)
         tuned = solver.autotune(dt, forcing=forcing, damping=damping)
     state = solver.multistep(state, args.warmup, dt, forcing=forcing, damping=damping)
     torch.cuda.synchronize()
@@ -238,6 +243,7 @@ def main():
         "autotune": None if tuned is None else {
             "shapes": list(tuned[0]),
             "ms_per_pair": {f"{c},{r}": round(ms, 4) for (c, r), ms in tuned[1].items()}},
+        "shapes": list(plan.shape()[:2]),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
