@@ -207,12 +207,17 @@ def main():
         r = cpu_baseline.time_forced_scalars(N=N, n_steps=args.cpu_steps,
                                              workers=cpu_baseline.host_cores(), n_scalars=n_s,
                                              scheme=args.scheme, budget_s=25.0)
+        # the same path on one FFT worker (a shorter sample)
+        r1 = cpu_baseline.time_forced_scalars(N=N, n_steps=2, workers=1, n_scalars=n_s,
+                                              scheme=args.scheme, budget_s=10.0)
         cpu = {"value": N * N / r["sec_per_step"], "unit": "grid-point RK4 steps/s",
                "cores": r["workers"], "kind": "port",
                "sample": f"{r['steps']} timed steps (after 1 warm-up) of the same N={N} forced "
                          f"configuration with {n_s} scalars, NumPy/SciPy reference-structured "
                          f"c2c path, scipy.fft workers={r['workers']}, CPU {r['cpu']}",
-               "sec_per_step": r["sec_per_step"]}
+               "sec_per_step": r["sec_per_step"],
+               "single_worker": {"value": N * N / r1["sec_per_step"], "cores": 1,
+                                 "steps": r1["steps"], "sec_per_step": r1["sec_per_step"]}}
 
     out = {
         "metric": "N^2*RK4-steps/s + HBM GB/s at N=1024 fp64",
