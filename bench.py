@@ -311,6 +311,19 @@ def bench_single(args):
     finite = bool(torch.isfinite(st["theta_hat"]).all())
     sps = steps / sec
     B = step_bytes(N, n_s)
+    # live per-kernel timing (HIP events at the kernel boundaries, no graph)
+    from pygsquig_amd import _lib
+
+    if args.config == "plain":
+        plan = solver._plan(None, None)
+        prof = plan.profile_advance(dt, 20, _lib.KEY_EXPLICIT, 0, 0)
+    else:
+        plan = solver._plan(f, d)
+        prof = plan.profile_advance(dt, 2, _lib.KEY_DERIVED, f.seed, int(st["step"]))
+    lb = launch_bytes(N, n_s)
+    kernels = {k: {"avg_us": round(ms / cnt * 1e3, 2), "launches": cnt,
+                   "GBps": round(lb[k] / (ms / 1e3 / cnt) / 1e9, 1) if k in lb else None}
+               for k, (ms, cnt) in prof.items() if cnt}
     print(json.dumps({
         "metric": "N^2*RK4-steps/s + HBM GB/s fp64", "value": N * N * sps,
         "unit": "grid-point RK4 steps/s", "n_gpus": 1, "steps": steps, "warmup": warm,
@@ -323,6 +336,7 @@ def bench_single(args):
         "autotune": None if tuned is None else {
             "shapes": list(tuned[0]),
             "ms_per_pair": {f"{c},{r}": round(ms, 4) for (c, r), ms in tuned[1].items()}},
+        "kernels": kernels,
         "cpu_baseline": None,
         "note": "cpu_baseline is reported for the default configuration only"}))
 
