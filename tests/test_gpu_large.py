@@ -3,7 +3,8 @@ paths: one field per lockstep group, twiddles from the global table,
 1024-lane column blocks at N = 8192.  The oracle is too slow beyond N = 2048,
 so larger sizes are checked through size-independent properties: transform
 round trips and the exact decay of a single |k| shell (its Jacobian vanishes,
-so only the linear operator acts)."""
+so only the linear operator acts).  The forced N = 1024 benchmark workload is
+checked against the oracle step for step in every tile shape pair."""
 
 import numpy as np
 import pytest
@@ -88,3 +89,62 @@ def test_forced_scalar_step_vs_oracle_2048(pg):
     assert np.linalg.norm(got - sto["theta_hat"]) / np.linalg.norm(sto["theta_hat"]) < 1e-12
     gs = st["scalar_state"].scalars["a"].cpu().numpy()
     assert np.linalg.norm(gs - sc["a"]) / np.linalg.norm(sc["a"]) < 1e-12
+
+
+_F1024 = dict(N=1024, dt=5e-4, n=3, kf=40, seed=42)
+
+
+def _f1024_inputs():
+    N = _F1024["N"]
+    theta0 = ics.kolmogorov_53(N, seed=2)
+    blobs = {f"s{i}": ics.gaussian_blob(N, c, TWO_PI / 16)
+             for i, c in enumerate(ics.SCALAR_CENTRES[:2])}
+    return theta0, blobs
+
+
+@pytest.fixture(scope="module")
+def forced1024_oracle():
+    """Oracle states after n steps of the forced N = 1024 workload."""
+    from oracle import rng as orng
+
+    N, dt, n, kf = _F1024["N"], _F1024["dt"], _F1024["n"], _F1024["kf"]
+    nu_d = 1e-8 * (512 / N) ** 8
+    theta0, blobs = _f1024_inputs()
+    go = R.make_grid(N, TWO_PI)
+    so = R.Solver(go, 1.0, 0.0, 8, scheme="if")
+    fo = R.RingForcing(kf=kf, dk=2, epsilon=0.1)
+    do = R.CombinedDamping(mu=0.1, kf=kf, nu_p=nu_d, p=8)
+    sto = so.initialize(theta0)
+    sc = {k: R.fft2(v) for k, v in blobs.items()}
+    for i in range(n):
+        k = orng.step_key(_F1024["seed"], i)
+        sto = so.step(sto, dt, forcing=lambda y, k=k: fo(y, key=k, grid=go), damping=do)
+        sc = so.step_scalars(sc, sto["theta_hat"], dt, {k: 0.0 for k in blobs})
+    return sto["theta_hat"], sc
+
+
+@pytest.mark.parametrize("shapes", [(0, 0), (0, 1), (1, 1)])
+def test_forced_1024_workload_vs_oracle(pg, forced1024_oracle, shapes):
+    """The forced N = 1024 workload with two scalars (kf = 40, damping,
+    Gaussian scalars, IF, dt = 5e-4), in each column/row tile shape pair the
+    autotuner chooses from at N = 1024 (set on the solver's own plan)."""
+    from pygsquig_amd.scalars import gSQGSolverWithScalars
+
+    N, dt, n, kf = _F1024["N"], _F1024["dt"], _F1024["n"], _F1024["kf"]
+    nu_d = 1e-8 * (512 / N) ** 8
+    theta0, blobs = _f1024_inputs()
+    g = pg.make_grid(N, TWO_PI)
+    solver = gSQGSolverWithScalars(g, 1.0, 0.0, 8, scheme="if",
+                                   passive_scalars={k: {"kappa": 0.0} for k in blobs})
+    f = pg.RingForcing(kf=kf, dk=2, epsilon=0.1, seed=_F1024["seed"])
+    d = pg.CombinedDamping(mu=0.1, kf=kf, nu_p=nu_d, p=8)
+    solver._plan(f, d).set_shape(*shapes)
+    st = solver.initialize(torch.from_numpy(theta0).cuda(),
+                           scalar_init={k: torch.from_numpy(v) for k, v in blobs.items()})
+    st = solver.multistep(st, n, dt, forcing=f, damping=d)
+    tho, sco = forced1024_oracle
+    got = st["theta_hat"].cpu().numpy()
+    assert np.linalg.norm(got - tho) / np.linalg.norm(tho) < 1e-12
+    for k in blobs:
+        gs = st["scalar_state"].scalars[k].cpu().numpy()
+        assert np.linalg.norm(gs - sco[k]) / np.linalg.norm(sco[k]) < 1e-12, k
