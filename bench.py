@@ -114,7 +114,7 @@ def main():
                          "measure the spread between forcing realizations")
     ap.add_argument("--no-match-forcing", dest="match_forcing", action="store_false",
                     help="coarse forcing amplitude sqrt(eps) instead of sqrt(eps) (nc/nf)^2")
-    ap.add_argument("--closure-every", default="1",
+    ap.add_argument("--closure-every", default="8",
                     help="closure evaluated every M steps (0: every RK4 stage); a comma list "
                          "evaluates each (the first is reported)")
     args = ap.parse_args()
