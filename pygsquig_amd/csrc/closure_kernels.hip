@@ -107,15 +107,17 @@ __device__ __forceinline__ void stage_tile(uint16_t* li, const uint16_t* __restr
 // row, the reduction over the 27 (tap, channel) pairs k = 3 t + ci, padded
 // to 32 with zeros.  The forward epilogue (shared with the 64-channel
 // layers) is defined below.
+__device__ __forceinline__ int swz(int row, int chunk);
 __device__ __forceinline__ void store_fwd(const f32x16 (&acc)[2][2], const float* __restrict__ bias,
                                           uint16_t* zout, uint16_t* aout, size_t prow, int lr,
-                                          int lh);
+                                          int lh, uint16_t* stage, int l);
 
 __global__ void __launch_bounds__(256) conv_in_fwd(Feat f, const float* __restrict__ W0r,
                                                    const float* __restrict__ b0, uint16_t* z,
                                                    uint16_t* a, int H, int W) {
   __shared__ uint16_t xs[3][HR][HC];
   __shared__ __attribute__((aligned(16))) uint16_t wt[C * 32];  // [co][k]
+  __shared__ __attribute__((aligned(16))) uint16_t stage[TR * 64 * C];  // output staging
   const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63, lr = l & 31, lh = l >> 5;
   const int j0 = blockIdx.x * TC, i0 = blockIdx.y * TR, b = blockIdx.z;
   const long long img = f.sel ? f.sel[b] : b;
@@ -165,7 +167,7 @@ __global__ void __launch_bounds__(256) conv_in_fwd(Feat f, const float* __restri
       acc[h][1] = mfma_32x32x16(A, Bf[1], acc[h][1]);
     }
   }
-  store_fwd(acc, b0, z, a, ((size_t)b * H + i0 + wave) * W + j0, lr, lh);
+  store_fwd(acc, b0, z, a, ((size_t)b * H + i0 + wave) * W + j0, lr, lh, stage + wave * 64 * C, l);
 }
 
 // -------------------------------------------- 64 -> 64 layers (MFMA GEMM)
@@ -236,36 +238,81 @@ __global__ void __launch_bounds__(256, 2) conv_mid(const uint16_t* __restrict__ 
   }
   // epilogue straight from the accumulators
   const size_t prow = (ib + i0 + wave) * W + j0;
-  if (MODE == 0) {
-    store_fwd(acc, bias, zout, aout, prow, lr, lh);
+  if (MODE == 0) {  // the halo tile's LDS becomes the output stage (first barrier inside)
+    store_fwd(acc, bias, zout, aout, prow, lr, lh, li + wave * 64 * C, l);
     return;
   }
+  // Backward data: dz = bf16(acc * gelu'(zprev)).  zprev comes in and dz
+  // goes out through the wave's 8 KB of the (now free) halo tile's LDS, as
+  // whole 16-byte chunks of eight consecutive pixels per instruction; each
+  // lane updates its own 8-byte pieces in place in between.
+  uint16_t* stage = li + wave * 64 * C;
+  __syncthreads();  // every wave is done with the halo tile
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int idx = 64 * k + l, px = idx >> 3, ch = idx & 7;
+    *reinterpret_cast<u32x4*>(stage + swz(px, ch)) =
+        *reinterpret_cast<const u32x4*>(zprev + (prow + px) * C + 8 * ch);
+  }
+  __syncthreads();
 #pragma unroll
   for (int h = 0; h < 2; ++h)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int co0 = 32 * h + 8 * q + 4 * lh;
+    for (int q = 0; q < 4; ++q)
 #pragma unroll
       for (int g = 0; g < 2; ++g) {
-        const size_t o = (prow + 32 * g + lr) * C + co0;
-        const uint32_t* zq = reinterpret_cast<const uint32_t*>(zprev + o);
-        const uint32_t p0 = zq[0], p1 = zq[1];
+        uint32_t* sp = reinterpret_cast<uint32_t*>(stage + swz(32 * g + lr, 4 * h + q) + 4 * lh);
+        const uint32_t p0 = sp[0], p1 = sp[1];
         const f32x16& v = acc[h][g];
-        uint32_t* ap = reinterpret_cast<uint32_t*>(aout + o);
-        ap[0] = bf16x2_rne(v[4 * q] * gelu_d(bf2f(p0 & 0xffffu)),
+        sp[0] = bf16x2_rne(v[4 * q] * gelu_d(bf2f(p0 & 0xffffu)),
                            v[4 * q + 1] * gelu_d(bf2f(p0 >> 16)));
-        ap[1] = bf16x2_rne(v[4 * q + 2] * gelu_d(bf2f(p1 & 0xffffu)),
+        sp[1] = bf16x2_rne(v[4 * q + 2] * gelu_d(bf2f(p1 & 0xffffu)),
                            v[4 * q + 3] * gelu_d(bf2f(p1 >> 16)));
       }
-    }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int idx = 64 * k + l, px = idx >> 3, ch = idx & 7;
+    *reinterpret_cast<u32x4*>(aout + (prow + px) * C + 8 * ch) =
+        *reinterpret_cast<const u32x4*>(stage + swz(px, ch));
+  }
 }
 
 // Forward epilogue of a 64-channel x 64-pixel wave tile (rows of `acc`:
 // channels 32 h + 8 q + 4 lh + e, columns: pixels 32 g + lr of the image row
 // starting at pixel index prow): z = bf16(acc + bias), a = bf16(gelu(z)).
+// Each tile goes through the wave's 8 KB of LDS (`stage`, pixel rows of 64
+// channels, swizzled like the input tile) so that the global stores are
+// whole 16-byte chunks, eight consecutive pixels (1 KB) per instruction.
+// Every wave of the block calls this together: it holds block barriers, and
+// the caller's LDS reads must be complete.  Without z (inference) the
+// activations alone are stored straight from the registers, 8 bytes per lane,
+// which measured faster than the staged path for a single output.
 __device__ __forceinline__ void store_fwd(const f32x16 (&acc)[2][2], const float* __restrict__ bias,
                                           uint16_t* zout, uint16_t* aout, size_t prow, int lr,
-                                          int lh) {
+                                          int lh, uint16_t* stage, int l) {
+  if (!zout) {  // uniform over the block
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int co0 = 32 * h + 8 * q + 4 * lh;
+        float bb[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) bb[e] = bias[co0 + e];
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+          const f32x16& v = acc[h][g];
+          const uint32_t z0 = bf16x2_rne(v[4 * q] + bb[0], v[4 * q + 1] + bb[1]);
+          const uint32_t z1 = bf16x2_rne(v[4 * q + 2] + bb[2], v[4 * q + 3] + bb[3]);
+          uint32_t* ap = reinterpret_cast<uint32_t*>(aout + (prow + 32 * g + lr) * C + co0);
+          ap[0] = bf16x2_rne(gelu(bf2f(z0 & 0xffffu)), gelu(bf2f(z0 >> 16)));
+          ap[1] = bf16x2_rne(gelu(bf2f(z1 & 0xffffu)), gelu(bf2f(z1 >> 16)));
+        }
+      }
+    return;
+  }
+  uint32_t zb[2][4][2][2];  // [h][q][g][pair]
 #pragma unroll
   for (int h = 0; h < 2; ++h)
 #pragma unroll
@@ -276,20 +323,39 @@ __device__ __forceinline__ void store_fwd(const f32x16 (&acc)[2][2], const float
       for (int e = 0; e < 4; ++e) bb[e] = bias[co0 + e];
 #pragma unroll
       for (int g = 0; g < 2; ++g) {
-        const size_t o = (prow + 32 * g + lr) * C + co0;
         const f32x16& v = acc[h][g];
-        const uint32_t z0 = bf16x2_rne(v[4 * q] + bb[0], v[4 * q + 1] + bb[1]);
-        const uint32_t z1 = bf16x2_rne(v[4 * q + 2] + bb[2], v[4 * q + 3] + bb[3]);
-        if (zout) {
-          uint32_t* zp = reinterpret_cast<uint32_t*>(zout + o);
-          zp[0] = z0;
-          zp[1] = z1;
-        }
-        uint32_t* ap = reinterpret_cast<uint32_t*>(aout + o);
-        ap[0] = bf16x2_rne(gelu(bf2f(z0 & 0xffffu)), gelu(bf2f(z0 >> 16)));
-        ap[1] = bf16x2_rne(gelu(bf2f(z1 & 0xffffu)), gelu(bf2f(z1 >> 16)));
+        zb[h][q][g][0] = bf16x2_rne(v[4 * q] + bb[0], v[4 * q + 1] + bb[1]);
+        zb[h][q][g][1] = bf16x2_rne(v[4 * q + 2] + bb[2], v[4 * q + 3] + bb[3]);
       }
     }
+#pragma unroll
+  for (int pass = 0; pass < 2; ++pass) {
+    uint16_t* dst = pass == 0 ? zout : aout;
+    __syncthreads();  // the stage is free
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+          uint32_t w0 = zb[h][q][g][0], w1 = zb[h][q][g][1];
+          if (pass == 1) {
+            w0 = bf16x2_rne(gelu(bf2f(w0 & 0xffffu)), gelu(bf2f(w0 >> 16)));
+            w1 = bf16x2_rne(gelu(bf2f(w1 & 0xffffu)), gelu(bf2f(w1 >> 16)));
+          }
+          const int px = 32 * g + lr, ch = 4 * h + q;  // 16-byte chunk of the pixel row
+          uint32_t* sp = reinterpret_cast<uint32_t*>(stage + swz(px, ch) + 4 * lh);
+          sp[0] = w0;
+          sp[1] = w1;
+        }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int idx = 64 * k + l, px = idx >> 3, ch = idx & 7;
+      *reinterpret_cast<u32x4*>(dst + (prow + px) * C + 8 * ch) =
+          *reinterpret_cast<const u32x4*>(stage + swz(px, ch));
+    }
+  }
 }
 
 // --------------------------------------------------- layer 5 forward
