@@ -53,7 +53,12 @@ struct RKShape {
   static constexpr int WX = W0 < N ? W0 : N;  // x-rows per block
   static constexpr int NT = WX * Q;
   static constexpr int PADM = Padded<M>::value;
-  static constexpr int REGION = WX * PADM;  // LDS elements per field tile
+  // Stride between the x-lines of a tile.  Where a wave holds one x-line
+  // (Q >= 64) the transforms never mix lines, and 8 / WX extra elements
+  // spread the transposing accesses (WX lines x 64 / WX rows per
+  // instruction) over the LDS banks instead of stacking them PADM apart.
+  static constexpr int RSTR = PADM + ((Q >= 64 && WX > 1 && WX <= 8) ? 8 / WX : 0);
+  static constexpr int REGION = WX * RSTR;  // LDS elements per field tile
   // Field tiles transformed together (two where they fit in 80 KB of LDS;
   // see CKShape::FP).
   static constexpr int FP = REGION * 16 * 2 <= 80 * 1024 ? 2 : 1;
@@ -788,13 +793,13 @@ template <int N, int V, int G, class PT>
 __device__ __forceinline__ void rk_commit(const vd2 (&tmp)[G * 8], cd* lds, int tid, int b, int t,
                                           const PT& pre, cd (&v)[G][8]) {
   using S = RKShape<N, V>;
-  constexpr int M = S::M, WX = S::WX, NT = S::NT, PADM = S::PADM, Q = S::Q, RG = S::REGION;
+  constexpr int M = S::M, WX = S::WX, NT = S::NT, Q = S::Q, RG = S::REGION;
   __syncthreads();
 #pragma unroll
   for (int e = 0; e < G * 8; ++e) {
     const int l = tid + (e & 7) * NT;
     const int jr = l / WX, bb = l - jr * WX;
-    *reinterpret_cast<vd2*>(lds + (e >> 3) * RG + bb * PADM + lpad(jr)) = tmp[e];
+    *reinterpret_cast<vd2*>(lds + (e >> 3) * RG + bb * S::RSTR + lpad(jr)) = tmp[e];
   }
   __syncthreads();
 #pragma unroll
@@ -803,7 +808,7 @@ __device__ __forceinline__ void rk_commit(const vd2 (&tmp)[G * 8], cd* lds, int 
     const cd w = cconj(pre(k));
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-      const cd* buf = lds + g * RG + b * PADM;
+      const cd* buf = lds + g * RG + b * S::RSTR;
       const cd X = buf[lpad(k)];
       if (k == 0) {
         v[g][m] = cmk(X.x + X.y, X.x - X.y);
@@ -821,8 +826,8 @@ template <int N, int V, class PT>
 __device__ __forceinline__ void rk_store_r2c(cd (&v)[8], cd* dst, int x0, cd* lds, int tid, int b,
                                              int t, const PT& pre) {
   using S = RKShape<N, V>;
-  constexpr int M = S::M, WX = S::WX, NT = S::NT, PADM = S::PADM, Q = S::Q;
-  cd* buf = lds + b * PADM;
+  constexpr int M = S::M, WX = S::WX, NT = S::NT, Q = S::Q;
+  cd* buf = lds + b * S::RSTR;
   __syncthreads();
 #pragma unroll
   for (int m = 0; m < 8; ++m) buf[lpad(t + m * Q)] = v[m];
@@ -831,7 +836,7 @@ __device__ __forceinline__ void rk_store_r2c(cd (&v)[8], cd* dst, int x0, cd* ld
   for (int r = 0; r < 8; ++r) {
     const int l = tid + r * NT;
     const int jr = l / WX, bb = l - jr * WX;
-    const cd* bf = lds + bb * PADM;
+    const cd* bf = lds + bb * S::RSTR;
     const cd Z = bf[lpad(jr)];
     cd X;
     if (jr == 0) {
@@ -849,7 +854,7 @@ __device__ __forceinline__ void rk_store_r2c(cd (&v)[8], cd* dst, int x0, cd* ld
 template <int N, int JOB, int V>
 __global__ void __launch_bounds__((RKShape<N, V>::NT)) rk_kernel(RKArgs a) {
   using S = RKShape<N, V>;
-  constexpr int M = S::M, Q = S::Q, WX = S::WX, NT = S::NT, PADM = S::PADM, RG = S::REGION;
+  constexpr int M = S::M, Q = S::Q, WX = S::WX, NT = S::NT, RG = S::REGION;
   constexpr int NF = RKFields<JOB>::value;
   constexpr int G = S::FP < NF ? S::FP : NF;  // fields per lockstep group
   constexpr bool THETA = (JOB == RK_THETA || JOB == RK_THETA_CL);
@@ -866,7 +871,7 @@ __global__ void __launch_bounds__((RKShape<N, V>::NT)) rk_kernel(RKArgs a) {
   const int x = x0 + b;
   const size_t fo = (size_t)blockIdx.y * (size_t)a.field_stride;
   const cd* __restrict__ tw = a.tw;
-  cd* buf = lds + b * PADM;
+  cd* buf = lds + b * S::RSTR;
 
   // Reduce the <theta f> partials of stage red_slot (block 0 only; every
   // other block is independent of it).
