@@ -157,16 +157,22 @@ struct TwGlobal {
 // never wait behind the block's outstanding global stores).  W^k for any
 // 0 <= k < LT follows exactly from the quarter turn W^(LT/4) = -i and
 // W^(LT/4 - r) = -i conj(W^r): every value equals the full table's entry.
+//
+// Entry r is stored at r + r/8: the radix-8 passes read entries in strides
+// of 2^n, which without the pad fall into one LDS bank (and serialise); the
+// pad spreads every such set of eight over distinct banks.
 template <int LT>
 struct OctTab {
-  static constexpr int SIZE = LT / 8 + 1;
+  static constexpr int ENTRIES = LT / 8 + 1;
+  static constexpr int SIZE = ENTRIES + ENTRIES / 8 + 1;  // LDS elements
   const cd* t;
+  __device__ __forceinline__ static int slot(int r) { return r + (r >> 3); }
   __device__ __forceinline__ cd fwd(int k) const {
     constexpr int QL = LT / 4;
     const int q = k / QL;
     const int r = k - q * QL;
     const bool lo = r <= LT / 8;
-    const cd c = t[lo ? r : QL - r];
+    const cd c = t[slot(lo ? r : QL - r)];
     const cd w = lo ? c : cmk(-c.y, -c.x);
     if (q == 0) return w;
     if (q == 1) return cmk(w.y, -w.x);
@@ -174,7 +180,7 @@ struct OctTab {
     return cmk(-w.y, w.x);
   }
   __device__ __forceinline__ static void fill(cd* dst, const cd* __restrict__ tw, int tid, int nt) {
-    for (int e = tid; e < SIZE; e += nt) dst[e] = tw[LT + e];
+    for (int e = tid; e < ENTRIES; e += nt) dst[slot(e)] = tw[LT + e];
   }
 };
 
