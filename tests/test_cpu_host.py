@@ -105,3 +105,16 @@ def test_solver_validation_messages():
         pg.gSQGSolver(FakeGrid(), 1.0, p=3)
     with pytest.raises(ValueError, match="p must be 2, 4, or 8"):
         pg.CombinedDamping(p=5)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; --gpus N > 1 exits with a message (no GPU
+    is touched before the check)."""
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr
