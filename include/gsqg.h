@@ -134,6 +134,14 @@ int gsqg_store_scalar(gsqg_plan* plan, int idx, void* scalar_hat_dev, void* stre
 int gsqg_advance(gsqg_plan* plan, double dt, int n_steps, int key_mode, uint64_t key,
                  int64_t step0, int use_graph, void* stream);
 
+/* Capture the HIP graphs gsqg_advance replays for (dt, key_mode, key) --
+ * the 16-step graph and the one-step graph -- without running them: the
+ * plan's state is unchanged, and a following gsqg_advance with the same
+ * dt, key mode and key only replays.  Counterpart of the jit/lax.scan
+ * compilation of gSQGSolverOptimized.multistep (core/solver_optimized.py:
+ * 168-199), which the reference pays on its first call. */
+int gsqg_graph_capture(gsqg_plan* plan, double dt, int key_mode, uint64_t key);
+
 /* One RK4 step of every passive scalar of the plan with the given frozen
  * physical velocity (u_dev, v_dev: (n, n) float64).
  * Replaces MultiSpeciesEvolver.step / PassiveScalarEvolver.step

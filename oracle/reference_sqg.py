@@ -190,21 +190,24 @@ def hyperviscosity(theta_hat, grid: Grid, nu_p: float, p: int):
 # ---------------------------------------------------------------------------
 
 
-def core_rhs(theta_hat, grid: Grid, alpha: float, nu_p: float, p: int):
-    """solver.py:28-58."""
+def advection_hat(theta_hat, grid: Grid, alpha: float):
+    """The advective part of the RHS, fft2(-J(theta, u, v)) with (u, v) from
+    theta_hat itself (solver.py:46-53).  Shared by core_rhs, the optimized
+    RHS and the integrating-factor scheme's nonlinear term."""
     theta = ifft2(theta_hat)
     u, v = compute_velocity_from_theta(theta_hat, grid, alpha)
     advection = -jacobian(theta, u, v, grid)
-    adv_hat = fft2(advection)
-    return adv_hat + hyperviscosity(theta_hat, grid, nu_p, p)
+    return fft2(advection)
+
+
+def core_rhs(theta_hat, grid: Grid, alpha: float, nu_p: float, p: int):
+    """solver.py:28-58."""
+    return advection_hat(theta_hat, grid, alpha) + hyperviscosity(theta_hat, grid, nu_p, p)
 
 
 def core_rhs_optimized(theta_hat, grid: Grid, alpha: float, nu_p: float, p: int):
     """solver_optimized.py:26-45 (dissipation without the [0,0] zeroing)."""
-    theta = ifft2(theta_hat)
-    u, v = compute_velocity_from_theta(theta_hat, grid, alpha)
-    adv_hat = fft2(-jacobian(theta, u, v, grid))
-    return adv_hat + (-nu_p * (grid.k2 ** (p / 2)) * theta_hat)
+    return advection_hat(theta_hat, grid, alpha) + (-nu_p * (grid.k2 ** (p / 2)) * theta_hat)
 
 
 def rk4_step(state, rhs_fn: Callable, dt: float):
@@ -350,10 +353,14 @@ class CombinedDamping:
 # ---------------------------------------------------------------------------
 
 
+def scalar_advection_hat(scalar_hat, u, v, grid: Grid):
+    """fft2(-J(s, u, v)) with a given (frozen) velocity (passive_scalar.py:86-95)."""
+    return fft2(-jacobian(ifft2(scalar_hat), u, v, grid))
+
+
 def scalar_rhs(scalar_hat, u, v, grid: Grid, kappa: float):
     """passive_scalar.py:63-103 without a source term."""
-    s = ifft2(scalar_hat)
-    adv_hat = fft2(-jacobian(s, u, v, grid))
+    adv_hat = scalar_advection_hat(scalar_hat, u, v, grid)
     diff = -kappa * grid.k2 * scalar_hat
     return adv_hat + diff
 
@@ -366,7 +373,12 @@ def scalar_rhs(scalar_hat, u, v, grid: Grid, kappa: float):
 class Solver:
     """gSQGSolver restated.  ``scheme='explicit'`` is the reference's;
     ``scheme='if'`` integrates the linear terms (solver hyperviscosity and
-    the damping object's terms) exactly."""
+    the damping object's terms) exactly.
+
+    ``combine='multistep'`` restates gSQGSolverOptimized
+    (solver_optimized.py:83-199): its RHS never zeroes the hyperviscosity at
+    k = 0 (:42-45), its unforced step combines as ``(dt/6.0)*(...)`` (:62),
+    and with forcing or damping it falls back to ``rk4_step`` (:152-164)."""
 
     def __init__(self, grid: Grid, alpha: float, nu_p: float = 0.0, p: int = 8,
                  scheme: str = "explicit", combine: str = "step"):
@@ -388,7 +400,7 @@ class Solver:
 
     def compute_rhs(self, theta_hat, forcing=None, damping=None):
         """solver.py:150-180 with per-stage closures."""
-        if self.combine == "multistep" and forcing is None and damping is None:
+        if self.combine == "multistep":
             rhs = core_rhs_optimized(theta_hat, self.grid, self.alpha, self.nu_p, self.p)
         else:
             rhs = core_rhs(theta_hat, self.grid, self.alpha, self.nu_p, self.p)
@@ -424,9 +436,7 @@ class Solver:
             g = self.grid
 
             def nonlin(y):
-                theta = ifft2(y)
-                u, v = compute_velocity_from_theta(y, g, self.alpha)
-                r = fft2(-jacobian(theta, u, v, g))
+                r = advection_hat(y, g, self.alpha)
                 if forcing is not None:
                     r = r + forcing(y)
                 return r
@@ -434,7 +444,7 @@ class Solver:
             new = if_rk4_step(th, nonlin, lin, dt)
         else:
             fn = lambda y: self.compute_rhs(y, forcing, damping)  # noqa: E731
-            if self.combine == "multistep":
+            if self.combine == "multistep" and forcing is None and damping is None:
                 new = rk4_step_multistep_order(th, fn, dt)
             else:
                 new = rk4_step(th, fn, dt)
@@ -454,7 +464,7 @@ class Solver:
             kap = kappas.get(name, 0.0)
             if self.scheme == "if":
                 out[name] = if_rk4_step(
-                    s_hat, lambda s: fft2(-jacobian(ifft2(s), u, v, g)), -kap * g.k2, dt)
+                    s_hat, lambda s: scalar_advection_hat(s, u, v, g), -kap * g.k2, dt)
             else:
                 out[name] = rk4_step(s_hat, lambda s: scalar_rhs(s, u, v, g, kap), dt)
         return out

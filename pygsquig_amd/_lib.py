@@ -90,6 +90,7 @@ _SIGS = {
     "gsqg_store_scalar": (ctypes.c_int, [_VP, ctypes.c_int, _VP, _VP]),
     "gsqg_advance": (ctypes.c_int, [_VP, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_uint64, ctypes.c_int64, ctypes.c_int, _VP]),
+    "gsqg_graph_capture": (ctypes.c_int, [_VP, ctypes.c_double, ctypes.c_int, ctypes.c_uint64]),
     "gsqg_advance_scalars": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_double, _VP]),
     "gsqg_rhs": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_uint64, _VP]),
     "gsqg_velocity": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP]),

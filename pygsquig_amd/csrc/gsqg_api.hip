@@ -607,6 +607,38 @@ static int capture_steps(gsqg_plan* p, int nsteps, double dt, bool derived, bool
   return GSQG_OK;
 }
 
+// The cached graphs replay steps of one (dt, key mode, key); other values
+// drop them.  Captures the multi-step graph (need_k) and the one-step graph
+// (need_1) where missing.  Capture records launches without running them, so
+// the plan's state is untouched.
+static int ensure_graphs(gsqg_plan* p, double dt, int key_mode, uint64_t key, bool need_k,
+                         bool need_1) {
+  if (p->g_dt != dt || p->g_key_mode != key_mode || p->g_key != key) {
+    if (p->gexec) HIPCHK(hipGraphExecDestroy(p->gexec));
+    if (p->gexec_k) HIPCHK(hipGraphExecDestroy(p->gexec_k));
+    p->gexec = p->gexec_k = nullptr;
+    p->g_dt = dt;
+    p->g_key_mode = key_mode;
+    p->g_key = key;
+  }
+  const bool derived = key_mode == GSQG_KEY_DERIVED;
+  const bool sc = p->n_scalars > 0;
+  int rc;
+  if (need_k && !p->gexec_k)
+    if ((rc = capture_steps(p, kGraphSteps, dt, derived, sc, &p->gexec_k))) return rc;
+  if (need_1 && !p->gexec)
+    if ((rc = capture_steps(p, 1, dt, derived, sc, &p->gexec))) return rc;
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_graph_capture(gsqg_plan* p, double dt, int key_mode, uint64_t key) {
+  if (!p) return fail(GSQG_ERR_ARG, "null plan");
+  if (key_mode != GSQG_KEY_EXPLICIT && key_mode != GSQG_KEY_DERIVED)
+    return fail(GSQG_ERR_ARG, "key_mode must be GSQG_KEY_EXPLICIT or GSQG_KEY_DERIVED");
+  const bool graph1 = !(p->cl_net && p->cl_every > 1);
+  return ensure_graphs(p, dt, key_mode, key, true, graph1);
+}
+
 extern "C" int gsqg_advance(gsqg_plan* p, double dt, int n_steps, int key_mode, uint64_t key,
                             int64_t step0, int use_graph, void* stream) {
   if (!p) return fail(GSQG_ERR_ARG, "null plan");
@@ -633,22 +665,11 @@ extern "C" int gsqg_advance(gsqg_plan* p, double dt, int n_steps, int key_mode, 
   const int n_mid = n_steps >= 2 ? n_steps - 2 : 0;
   if (n_mid > 0) {
     if (use_graph && n_mid >= 2) {
-      if (p->g_dt != dt || p->g_key_mode != key_mode || p->g_key != key) {
-        if (p->gexec) HIPCHK(hipGraphExecDestroy(p->gexec));
-        if (p->gexec_k) HIPCHK(hipGraphExecDestroy(p->gexec_k));
-        p->gexec = p->gexec_k = nullptr;
-        p->g_dt = dt;
-        p->g_key_mode = key_mode;
-        p->g_key = key;
-      }
       const int nk = n_mid / kGraphSteps, n1 = n_mid % kGraphSteps;
       // a one-step graph repeats one evaluation pattern: with a closure held
       // over several steps the remainder runs eagerly
       const bool graph1 = !(p->cl_net && p->cl_every > 1);
-      if (nk > 0 && !p->gexec_k)
-        if ((rc = capture_steps(p, kGraphSteps, dt, derived, sc, &p->gexec_k))) return rc;
-      if (n1 > 0 && graph1 && !p->gexec)
-        if ((rc = capture_steps(p, 1, dt, derived, sc, &p->gexec))) return rc;
+      if ((rc = ensure_graphs(p, dt, key_mode, key, nk > 0, n1 > 0 && graph1))) return rc;
       for (int s = 0; s < nk; ++s) HIPCHK(hipGraphLaunch(p->gexec_k, st));
       for (int s = 0; s < n1; ++s) {
         if (graph1) {

@@ -4,7 +4,8 @@
 reference's operation order (core/grid.py:72-107) and keeps both the host
 (NumPy) and device (torch) copies: the host copies feed the plan's per-mode
 flag table, the device tensors are the ``Grid`` attributes users read.
-``fft2``/``ifft2`` run the library's R2C/C2R kernels.
+``fft2``/``ifft2`` run the library's R2C/C2R kernels and take the
+reference's arguments, ``fft2(field)`` and ``ifft2(field_hat)``.
 """
 
 from __future__ import annotations
@@ -85,34 +86,60 @@ def make_grid(N: int, L: float, device: Optional[torch.device] = None) -> Grid:
     return Grid(N, L, t(x), t(y), t(kx), t(ky), t(k2), t(mask), k1d)
 
 
-def _as_device(a, grid: Grid, dtype) -> torch.Tensor:
-    if isinstance(a, torch.Tensor):
-        t = a
+def _square_n(shape) -> int:
+    if len(shape) != 2 or shape[0] != shape[1]:
+        raise ValueError(f"expected a square (N, N) field, got shape {tuple(shape)}")
+    return int(shape[0])
+
+
+def _as_device(a, grid: Optional[Grid], dtype) -> torch.Tensor:
+    """``a`` as a contiguous device tensor of ``dtype``: on ``grid``'s
+    device when a grid is given, else on the tensor's own GPU (or the
+    current GPU for host data)."""
+    t = a if isinstance(a, torch.Tensor) else torch.as_tensor(np.asarray(a))
+    if grid is not None:
+        dev = grid.device
+        if t.shape != (grid.N, grid.N):
+            raise ValueError(f"expected shape {(grid.N, grid.N)}, got {tuple(t.shape)}")
     else:
-        t = torch.as_tensor(np.asarray(a))
-    t = t.to(device=grid.device, dtype=dtype)
-    if t.shape != (grid.N, grid.N):
-        raise ValueError(f"expected shape {(grid.N, grid.N)}, got {tuple(t.shape)}")
-    return t.contiguous()
+        _square_n(t.shape)
+        dev = t.device if t.is_cuda else default_device()
+    return t.to(device=dev, dtype=dtype).contiguous()
 
 
-def fft2(field, grid: Grid) -> torch.Tensor:
-    """Unnormalised 2D FFT of a real field -> full (N, N) complex128
-    (core/grid.py:110-124)."""
-    from .plan import transform_plan
+def _plan_for(N: int, device: torch.device, grid: Optional[Grid]):
+    from .plan import transform_plan, transform_plan_n
 
-    f = _as_device(field if not torch.is_tensor(field) or not field.is_complex() else field.real,
-                   grid, torch.float64)
-    out = torch.empty((grid.N, grid.N), dtype=torch.complex128, device=grid.device)
-    transform_plan(grid).fft2(f, out)
+    return transform_plan(grid) if grid is not None else transform_plan_n(N, device)
+
+
+def fft2(field, grid: Optional[Grid] = None) -> torch.Tensor:
+    """Unnormalised 2D FFT -> full (N, N) complex128 (core/grid.py:110-124).
+
+    Called as the reference calls it, ``fft2(field)``: N comes from the
+    field's shape (a transform does not depend on L) and the transform plan
+    is cached per (N, device).  ``grid`` may be passed to pin the device.
+    A real field runs one R2C transform; a complex field runs the full c2c
+    transform the reference's ``jnp.fft.fft2`` does, as
+    F(Re f) + i F(Im f) from two R2C transforms."""
+    if torch.is_tensor(field) and field.is_complex() or (
+            not torch.is_tensor(field) and np.iscomplexobj(field)):
+        z = _as_device(field, grid, torch.complex128)
+        return fft2(z.real.contiguous(), grid) + 1j * fft2(z.imag.contiguous(), grid)
+    f = _as_device(field, grid, torch.float64)
+    N = f.shape[0]
+    out = torch.empty((N, N), dtype=torch.complex128, device=f.device)
+    _plan_for(N, f.device, grid).fft2(f, out)
     return out
 
 
-def ifft2(field_hat, grid: Grid) -> torch.Tensor:
-    """``ifft2(field_hat).real`` (core/grid.py:127-138)."""
-    from .plan import transform_plan
-
+def ifft2(field_hat, grid: Optional[Grid] = None) -> torch.Tensor:
+    """``ifft2(field_hat).real`` (core/grid.py:127-138), also for a
+    non-Hermitian ``field_hat`` (the C2R runs on its Hermitian part, which is
+    exactly what ``.real`` of the c2c inverse keeps).  Called as
+    ``ifft2(field_hat)``; ``grid`` is optional as for ``fft2``."""
     X = _as_device(field_hat, grid, torch.complex128)
-    out = torch.empty((grid.N, grid.N), dtype=torch.float64, device=grid.device)
-    transform_plan(grid).ifft2(X, out)
+    N = X.shape[0]
+    out = torch.empty((N, N), dtype=torch.float64, device=X.device)
+    _plan_for(N, X.device, grid).ifft2(X, out)
     return out

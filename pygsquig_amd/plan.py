@@ -176,6 +176,14 @@ class Plan:
                                          int(step0), 1 if use_graph else 0, _stream()),
                    "gsqg_advance")
 
+    def graph_capture(self, dt: float, key_mode: int, key: int) -> None:
+        """Capture the replay graphs ``advance`` uses for (dt, key_mode, key)
+        without running them (gsqg_graph_capture); the state is unchanged."""
+        with torch.cuda.device(self.grid.device):
+            _lib.check(self.lib.gsqg_graph_capture(self.handle, float(dt), int(key_mode),
+                                                   ctypes.c_uint64(key & 0xFFFFFFFFFFFFFFFF)),
+                       "gsqg_graph_capture")
+
     def advance_scalars(self, u: torch.Tensor, v: torch.Tensor, dt: float) -> None:
         _lib.check(self.lib.gsqg_advance_scalars(self.handle, _ptr(u), _ptr(v), float(dt),
                                                  _stream()), "gsqg_advance_scalars")
@@ -300,11 +308,22 @@ def forcing_plan(grid: Grid, forcing) -> Plan:
     return p
 
 
-def transform_plan(grid: Grid) -> Plan:
-    """Shared plan for fft2/ifft2 (only N matters for pure transforms)."""
-    key = (grid.N, grid.L, str(grid.device))
+def transform_plan_n(N: int, device: torch.device) -> Plan:
+    """Shared plan for fft2/ifft2 of (N, N) fields on ``device``: a pure
+    transform depends on N only, so one plan serves every grid of that size."""
+    from .grid import make_grid
+
+    dev = torch.device(device)
+    if dev.index is None:
+        dev = torch.device(dev.type, torch.cuda.current_device())
+    key = (int(N), str(dev))
     p = _TRANSFORM_PLANS.get(key)
     if p is None:
-        p = Plan(grid, 1.0, 0.0, 8)
+        p = Plan(make_grid(int(N), 2 * np.pi, dev), 1.0, 0.0, 8)
         _TRANSFORM_PLANS[key] = p
     return p
+
+
+def transform_plan(grid: Grid) -> Plan:
+    """Shared plan for fft2/ifft2 on ``grid``'s size and device."""
+    return transform_plan_n(grid.N, grid.device)

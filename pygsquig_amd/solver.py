@@ -213,6 +213,14 @@ class gSQGSolver:
         self._remember(plan, out)
         return out
 
+    def _combine_for(self, forcing, damping) -> int:
+        """The RK4 combine of a step: the solver's own without plug-ins; the
+        reference's ``rk4_step`` order with them (core/solver_optimized.py:152-164
+        falls back to it)."""
+        if forcing is None and damping is None:
+            return self._combine
+        return _lib.COMBINE_STEP
+
     def step(self, state: State, dt: float, forcing: Optional[Callable] = None,
              damping: Optional[Callable] = None, **kwargs) -> State:
         """core/solver.py:182-214."""
@@ -220,14 +228,16 @@ class gSQGSolver:
         if generic:
             new = self._step_generic(state, dt, forcing, damping, kwargs)
         else:
-            new = self._advance_fused(state, dt, 1, fused_f, fused_d, kwargs.get("key"))
+            new = self._advance_fused(state, dt, 1, fused_f, fused_d, kwargs.get("key"),
+                                      combine=self._combine_for(forcing, damping))
         return {"theta_hat": new, "time": state["time"] + dt, "step": state["step"] + 1}
 
     def multistep(self, state: State, n_steps: int, dt: float, forcing=None, damping=None,
                   **kwargs) -> State:
         """n_steps device-resident steps (core/solver_optimized.py:168-199
         for the unforced case; forcing/damping objects are also accepted,
-        with per-step keys step_key(forcing.seed, step))."""
+        with per-step keys step_key(forcing.seed, step), and every step then
+        combines as ``step`` does, so ``multistep(n)`` equals n ``step`` calls)."""
         fused_f, fused_d, generic = self._split_plugins(forcing, damping, kwargs)
         if generic:
             s = state
@@ -235,11 +245,28 @@ class gSQGSolver:
                 s = self.step(s, dt, forcing, damping, **kwargs)
             return s
         new = self._advance_fused(state, dt, n_steps, fused_f, fused_d, kwargs.get("key"),
-                                  combine=kwargs.get("combine"))
+                                  combine=self._combine_for(forcing, damping))
         t = _scalar_tensor(state["time"], torch.float64)
         for _ in range(n_steps):
             t = t + dt
         return {"theta_hat": new, "time": t, "step": state["step"] + n_steps}
+
+    def capture_graphs(self, dt: float, forcing=None, damping=None, key=None) -> None:
+        """Capture the HIP graphs that ``multistep(state, n, dt, forcing,
+        damping[, key=key])`` replays, without running them
+        (gsqg_graph_capture): a later call with the same arguments only
+        replays.  The reference pays the equivalent cost, jit compilation of
+        its lax.scan, on its first multistep call.  Generic callables run no
+        graphs; nothing is captured for them."""
+        fused_f, fused_d, generic = self._split_plugins(forcing, damping, {})
+        if generic:
+            return
+        plan = self._plan(fused_f, fused_d, self._combine_for(forcing, damping))
+        if key is not None:
+            mode, k = _lib.KEY_EXPLICIT, key_to_u64(key)
+        else:
+            mode, k = _lib.KEY_DERIVED, (fused_f.seed if fused_f is not None else 0)
+        plan.graph_capture(dt, mode, k)
 
     def get_diagnostics(self, state: State) -> dict:
         """core/solver.py:216-245: kinetic energy 0.5 <u^2 + v^2>, enstrophy
@@ -259,7 +286,9 @@ class gSQGSolver:
 
 class gSQGSolverOptimized(gSQGSolver):
     """core/solver_optimized.py:83-226: the unforced step combines with
-    ``theta_hat + (dt/6.0)*(...)`` and hyperviscosity is not zeroed at k=0."""
+    ``theta_hat + (dt/6.0)*(...)`` (:62) and hyperviscosity is not zeroed at
+    k=0 (:42-45); with forcing or damping the step falls back to the
+    ``rk4_step`` combine (:152-164), still without the k=0 zeroing."""
 
     _combine = _lib.COMBINE_MULTISTEP
     _hv_dc_zero = False  # core/solver_optimized.py:42-43
@@ -267,26 +296,3 @@ class gSQGSolverOptimized(gSQGSolver):
     def __init__(self, grid: Grid, alpha: float, nu_p: float = 0.0, p: int = 8, **kw):
         super().__init__(grid, alpha, nu_p, p, **kw)
         self.params = {"alpha": alpha, "nu_p": nu_p, "p": float(p)}
-
-    def step(self, state, dt, forcing=None, damping=None, **kwargs):
-        if forcing is None and damping is None:
-            new = self._advance_fused(state, dt, 1, None, None, None)
-            return {"theta_hat": new, "time": state["time"] + dt, "step": state["step"] + 1}
-        # forced/damped path falls back to the standard combine (:152-164)
-        return gSQGSolver._step_with_combine(self, state, dt, forcing, damping, kwargs)
-
-    def _step_with_combine(self, state, dt, forcing, damping, kwargs):
-        fused_f, fused_d, generic = self._split_plugins(forcing, damping, kwargs)
-        if generic:
-            new = self._step_generic(state, dt, forcing, damping, kwargs)
-        else:
-            new = self._advance_fused(state, dt, 1, fused_f, fused_d, kwargs.get("key"),
-                                      combine=_lib.COMBINE_STEP)
-        return {"theta_hat": new, "time": state["time"] + dt, "step": state["step"] + 1}
-
-    def multistep(self, state, n_steps, dt, **kwargs):
-        """core/solver_optimized.py:168-199 (unforced lax.scan loop)."""
-        return super().multistep(state, n_steps, dt, **kwargs)
-
-
-gSQGSolver._step_with_combine = gSQGSolverOptimized._step_with_combine

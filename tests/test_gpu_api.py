@@ -1,0 +1,252 @@
+"""GPU tests of the Python boundary: the reference-arity transforms, the
+closed-form advection sign through the kernels, gSQGSolverOptimized against
+the oracle's restatement of core/solver_optimized.py, generic forcing and
+damping callables, scalar evolvers with a caller-supplied velocity, graph
+pre-capture and add_passive_scalar."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import ics
+from oracle import reference_sqg as R
+
+pytestmark = pytest.mark.gpu
+TWO_PI = 2 * np.pi
+
+
+def rel(a, b):
+    a = a.detach().cpu().numpy() if torch.is_tensor(a) else np.asarray(a)
+    b = b.detach().cpu().numpy() if torch.is_tensor(b) else np.asarray(b)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def pg():
+    import pygsquig_amd as pg
+
+    return pg
+
+
+# ------------------------------------------------------------- transforms
+@pytest.mark.parametrize("N", [32, 256])
+def test_fft2_ifft2_reference_arity(pg, N):
+    """fft2(field) / ifft2(field_hat) as core/grid.py:110-138 call them: no
+    grid argument, N from the shape; numpy input lands on the current GPU."""
+    print(f"[fft2 arity N={N}]", flush=True)
+    rs = np.random.default_rng(N + 1)
+    f = rs.standard_normal((N, N))
+    X = pg.fft2(torch.from_numpy(f).cuda())
+    assert X.is_cuda and X.dtype == torch.complex128
+    assert rel(X, np.fft.fft2(f)) < 1e-14
+    X2 = pg.fft2(f)  # host numpy input
+    assert X2.is_cuda and rel(X2, np.fft.fft2(f)) < 1e-14
+    assert rel(pg.ifft2(X), f) < 1e-14
+    Y = rs.standard_normal((N, N)) + 1j * rs.standard_normal((N, N))
+    assert rel(pg.ifft2(Y), np.fft.ifft2(Y).real) < 1e-14
+    with pytest.raises(ValueError):
+        pg.fft2(torch.zeros((N, N // 2), dtype=torch.float64, device="cuda"))
+
+
+@pytest.mark.parametrize("N", [16, 128])
+def test_fft2_complex_input_is_c2c(pg, N):
+    """A complex field gets the full c2c transform jnp.fft.fft2 computes."""
+    print(f"[fft2 complex N={N}]", flush=True)
+    rs = np.random.default_rng(N + 7)
+    z = rs.standard_normal((N, N)) + 1j * rs.standard_normal((N, N))
+    assert rel(pg.fft2(torch.from_numpy(z).cuda()), np.fft.fft2(z)) < 1e-14
+    assert rel(pg.fft2(z), np.fft.fft2(z)) < 1e-14
+    g = pg.make_grid(N, TWO_PI)
+    assert rel(pg.fft2(torch.from_numpy(z).cuda(), g), np.fft.fft2(z)) < 1e-14
+
+
+# -------------------------------------------------------- advection sign
+@pytest.mark.parametrize("alpha", [0.5, 1.0, 2.0, -1.0])
+def test_rhs_two_shell_closed_form(pg, alpha):
+    """theta = cos x + cos 2y: the inviscid RHS is -(2 - 2^(1-a)) sin x sin 2y
+    (core/solver.py:52), checked against the closed form, not the oracle."""
+    print(f"[rhs closed form alpha={alpha}]", flush=True)
+    N = 64
+    g = pg.make_grid(N, TWO_PI)
+    x, y = g.x, g.y
+    theta = torch.cos(x) + torch.cos(2 * y)
+    s = pg.gSQGSolver(g, alpha, 0.0, 8)
+    rhs = pg.ifft2(s.compute_rhs(pg.fft2(theta)))
+    exact = -(2.0 - 2.0 ** (1.0 - alpha)) * torch.sin(x) * torch.sin(2 * y)
+    assert float(torch.max(torch.abs(rhs - exact))) < 1e-12
+
+
+def test_scalar_caller_velocity_closed_form(pg):
+    """MultiSpeciesEvolver.step with a caller-supplied (u, v) = (sin 2y,
+    -sin x) on s = cos x: (s1 - s0)/dt -> +sin x sin 2y
+    (scalars/passive_scalar.py:86-95, 341-376)."""
+    from pygsquig_amd.scalars import MultiSpeciesEvolver
+
+    N, dt = 64, 1e-7
+    g = pg.make_grid(N, TWO_PI)
+    ev = MultiSpeciesEvolver(grid=g, species={"c": {"kappa": 0.0}})
+    st = ev.initialize({"c": torch.cos(g.x)})
+    st1 = ev.step(st, dt, torch.sin(2 * g.y), -torch.sin(g.x))
+    d = (pg.ifft2(st1.scalars["c"]) - torch.cos(g.x)) / dt
+    exact = torch.sin(g.x) * torch.sin(2 * g.y)
+    assert float(torch.max(torch.abs(d - exact))) < 1e-5
+
+
+@pytest.mark.parametrize("kappa", [0.0, 2e-3])
+def test_scalar_evolvers_caller_velocity_vs_oracle(pg, kappa):
+    """PassiveScalarEvolver.step and MultiSpeciesEvolver.step (two species)
+    with an arbitrary caller velocity (not a theta's) against the oracle's
+    rk4_step of scalar_rhs (gsqg_advance_scalars)."""
+    from pygsquig_amd.scalars import MultiSpeciesEvolver, PassiveScalarEvolver
+
+    print(f"[scalar evolvers kappa={kappa}]", flush=True)
+    N, dt = 64, 2e-3
+    g = pg.make_grid(N, TWO_PI)
+    go = R.make_grid(N, TWO_PI)
+    rs = np.random.default_rng(11)
+    u = 0.5 * ics.kolmogorov_53(N, seed=21)
+    v = 0.5 * ics.kolmogorov_53(N, seed=22)
+    s0 = ics.gaussian_blob(N, (np.pi, np.pi), TWO_PI / 16)
+    s1 = ics.gaussian_blob(N, (np.pi / 2, 3 * np.pi / 2), TWO_PI / 16) + 0.1 * rs.standard_normal((N, N))
+    want0 = R.rk4_step(R.fft2(s0), lambda s: R.scalar_rhs(s, u, v, go, kappa), dt)
+    want1 = R.rk4_step(R.fft2(s1), lambda s: R.scalar_rhs(s, u, v, go, kappa), dt)
+    ud, vd = torch.from_numpy(u).cuda(), torch.from_numpy(v).cuda()
+    ev = PassiveScalarEvolver(grid=g, kappa=kappa)
+    st = ev.initialize(scalar0=torch.from_numpy(s0).cuda())
+    out = ev.step(st, dt, ud, vd)
+    assert rel(out.scalar_hat, want0) < 1e-12
+    mev = MultiSpeciesEvolver(grid=g, species={"a": {"kappa": kappa}, "b": {"kappa": kappa}})
+    mst = mev.initialize({"a": torch.from_numpy(s0).cuda(), "b": torch.from_numpy(s1).cuda()})
+    mout = mev.step(mst, dt, ud, vd)
+    assert rel(mout.scalars["a"], want0) < 1e-12
+    assert rel(mout.scalars["b"], want1) < 1e-12
+
+
+# ------------------------------------------ gSQGSolverOptimized (A17)
+@pytest.mark.parametrize("N,alpha", [(64, 1.0), (128, 0.5)])
+def test_optimized_step_and_multistep_vs_oracle(pg, N, alpha):
+    """gSQGSolverOptimized.step and .multistep (core/solver_optimized.py:
+    48-62, 131-199: (dt/6.0) combine, no k=0 zeroing) against the oracle's
+    restatement; multistep(n) equals n optimized steps."""
+    print(f"[optimized N={N} alpha={alpha}]", flush=True)
+    g = pg.make_grid(N, TWO_PI)
+    go = R.make_grid(N, TWO_PI)
+    theta0 = 0.1 * ics.kolmogorov_53(N, seed=8, alpha=alpha)
+    dt, n = 1e-3, 20
+    s = pg.gSQGSolverOptimized(g, alpha, 1e-16, 8)
+    so = R.Solver(go, alpha, 1e-16, 8, combine="multistep")
+    st = s.initialize(torch.from_numpy(theta0).cuda())
+    sto = so.initialize(theta0)
+    a = st
+    for _ in range(n):
+        a = s.step(a, dt)
+        sto = so.step(sto, dt)
+    b = s.multistep(st, n, dt)
+    assert rel(a["theta_hat"], sto["theta_hat"]) < 1e-12
+    assert rel(b["theta_hat"], sto["theta_hat"]) < 1e-12
+    assert rel(b["theta_hat"], a["theta_hat"]) < 1e-14
+    assert int(b["step"]) == n and abs(float(b["time"]) - n * dt) < 1e-15
+
+
+def test_optimized_forced_step_vs_oracle(pg):
+    """The optimized solver's forced step falls back to rk4_step
+    (core/solver_optimized.py:152-164); multistep with the same forcing
+    combines every step the same way."""
+    N, dt = 64, 5e-4
+    g = pg.make_grid(N, TWO_PI)
+    go = R.make_grid(N, TWO_PI)
+    theta0 = 0.5 * ics.kolmogorov_53(N, seed=9)
+    f = pg.RingForcing(kf=8, dk=2, epsilon=0.1)
+    d = pg.CombinedDamping(mu=0.1, kf=8, nu_p=1e-10, p=8)
+    fo = R.RingForcing(kf=8, dk=2, epsilon=0.1)
+    do = R.CombinedDamping(mu=0.1, kf=8, nu_p=1e-10, p=8)
+    s = pg.gSQGSolverOptimized(g, 1.0, 1e-12, 8)
+    so = R.Solver(go, 1.0, 1e-12, 8, combine="multistep")
+    st = s.initialize(torch.from_numpy(theta0).cuda())
+    sto = so.initialize(theta0)
+    for k in range(4):
+        st = s.step(st, dt, forcing=f, damping=d, key=100 + k)
+        sto = so.step(sto, dt, forcing=lambda y, kk=100 + k: fo(y, key=kk, grid=go), damping=do)
+    assert rel(st["theta_hat"], sto["theta_hat"]) < 1e-12
+
+
+# ------------------------------------------------ generic plug-in callables
+def test_generic_callables_vs_oracle(pg):
+    """Arbitrary forcing and damping callables f(theta_hat, **kwargs), called
+    once per RK stage with the stage state (core/solver.py:173-178), through
+    the generic path; the kwargs reach the callables."""
+    N, dt = 64, 1e-3
+    g = pg.make_grid(N, TWO_PI)
+    go = R.make_grid(N, TWO_PI)
+    theta0 = 0.3 * ics.kolmogorov_53(N, seed=12)
+    rs = np.random.default_rng(4)
+    F0 = R.fft2(rs.standard_normal((N, N))) * (go.k2 < 36)
+    F0d = torch.from_numpy(F0).cuda()
+    calls = []
+
+    def forcing_t(th, amp=None, **kw):
+        calls.append(amp)
+        return amp * F0d - 0.05 * th
+
+    def damping_t(th, amp=None, **kw):
+        return -0.2 * torch.abs(th) * th * 1e-3
+
+    s = pg.gSQGSolver(g, 1.0, 1e-8, 4)
+    so = R.Solver(go, 1.0, 1e-8, 4)
+    st = s.initialize(torch.from_numpy(theta0).cuda())
+    sto = so.initialize(theta0)
+    for _ in range(3):
+        st = s.step(st, dt, forcing=forcing_t, damping=damping_t, amp=0.7)
+        sto = so.step(sto, dt, forcing=lambda y: 0.7 * F0 - 0.05 * y,
+                      damping=lambda y: -0.2 * np.abs(y) * y * 1e-3)
+    assert calls == [0.7] * 12  # four stages per step
+    assert rel(st["theta_hat"], sto["theta_hat"]) < 1e-12
+    assert int(st["step"]) == 3
+    # compute_rhs with the same callables
+    r = s.compute_rhs(torch.from_numpy(theta0).cuda(), forcing=forcing_t, damping=damping_t, amp=0.7)
+    ro = so.compute_rhs(R.fft2(theta0), forcing=lambda y: 0.7 * F0 - 0.05 * y,
+                        damping=lambda y: -0.2 * np.abs(y) * y * 1e-3)
+    assert rel(r, ro) < 1e-12
+
+
+# ------------------------------------------------------------ graphs, scalars
+def test_graph_capture_leaves_state_and_replays(pg):
+    """capture_graphs records the replay graphs without running them: a
+    multistep afterwards gives the same state as one without the capture."""
+    N, dt = 64, 5e-4
+    g = pg.make_grid(N, TWO_PI)
+    theta0 = torch.from_numpy(0.5 * ics.kolmogorov_53(N, seed=13)).cuda()
+    f = pg.RingForcing(kf=8, dk=2, epsilon=0.1, seed=5)
+    d = pg.CombinedDamping(mu=0.1, kf=8, nu_p=1e-10, p=8)
+    outs = []
+    for cap in (False, True):
+        s = pg.gSQGSolver(g, 1.0, 0.0, 8, scheme="if")
+        st = s.initialize(theta0)
+        if cap:
+            s.capture_graphs(dt, forcing=f, damping=d)
+            s.capture_graphs(dt, forcing=f, damping=d)  # idempotent
+        outs.append(s.multistep(st, 40, dt, forcing=f, damping=d)["theta_hat"])
+    assert torch.equal(outs[0], outs[1])
+
+
+def test_add_passive_scalar_vs_oracle(pg):
+    """add_passive_scalar (core/solver_with_scalars.py:212-236) registers a
+    scalar that then advances with the velocity of the new theta."""
+    from pygsquig_amd.scalars import gSQGSolverWithScalars
+
+    N, dt = 64, 1e-3
+    g = pg.make_grid(N, TWO_PI)
+    go = R.make_grid(N, TWO_PI)
+    theta0 = 0.3 * ics.kolmogorov_53(N, seed=14)
+    s0 = ics.gaussian_blob(N, (np.pi, np.pi), TWO_PI / 16)
+    s = gSQGSolverWithScalars(g, 1.0, 1e-16, 8)
+    s.add_passive_scalar("dye", kappa=1e-3)
+    with pytest.raises(ValueError):
+        s.add_passive_scalar("dye")
+    st = s.initialize(torch.from_numpy(theta0).cuda(), scalar_init={"dye": torch.from_numpy(s0)})
+    st = s.step(st, dt)
+    so = R.Solver(go, 1.0, 1e-16, 8)
+    th1 = so.step(so.initialize(theta0), dt)["theta_hat"]
+    sc1 = so.step_scalars({"dye": R.fft2(s0)}, th1, dt, {"dye": 1e-3})["dye"]
+    assert rel(st["theta_hat"], th1) < 1e-12
+    assert rel(st["scalar_state"].scalars["dye"], sc1) < 1e-12
