@@ -203,7 +203,8 @@ def test_generic_callables_vs_oracle(pg):
     assert rel(st["theta_hat"], sto["theta_hat"]) < 1e-12
     assert int(st["step"]) == 3
     # compute_rhs with the same callables
-    r = s.compute_rhs(torch.from_numpy(theta0).cuda(), forcing=forcing_t, damping=damping_t, amp=0.7)
+    r = s.compute_rhs(torch.from_numpy(R.fft2(theta0)).cuda(), forcing=forcing_t, damping=damping_t,
+                      amp=0.7)
     ro = so.compute_rhs(R.fft2(theta0), forcing=lambda y: 0.7 * F0 - 0.05 * y,
                         damping=lambda y: -0.2 * np.abs(y) * y * 1e-3)
     assert rel(r, ro) < 1e-12
