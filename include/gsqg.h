@@ -219,6 +219,21 @@ int gsqg_spectral_diagnostics(gsqg_plan* plan, const void* spectrum_dev, double 
                               const double* edges, int n_bins, double dk, double* bin_sums,
                               double* shell_sums, double* sums, void* stream);
 
+/* Allocation-free variant of gsqg_spectral_diagnostics: same arguments,
+ * kernels and results (bit for bit), but its scratch -- per-block partial
+ * sums, reduced slots, the device copy of `edges` -- lives in a file-scope
+ * device array of the library (17.9 MB per GPU, held until the process
+ * ends), so the call makes no hipMalloc/hipFree at all, on its first call
+ * included.  n_bins must be in [1, 256] (GSQG_ERR_ARG otherwise).  Calls
+ * that overlap on one GPU share the array and race; serialise them.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (tight diagnostic loops, serving loops, harnesses that
+ * forbid allocation after set-up). */
+int gsqg_spectral_diagnostics_noalloc(gsqg_plan* plan, const void* spectrum_dev, double alpha,
+                                      const double* edges, int n_bins, double dk,
+                                      double* bin_sums, double* shell_sums, double* sums,
+                                      void* stream);
+
 /* get_diagnostics (core/solver.py:216-245) of a full spectrum X with the
  * plan's alpha: out[0] = 0.5 mean(u^2 + v^2), out[1] = 0.5 mean(theta^2),
  * out[2] = max |theta|, theta = ifft2(X).real, (u, v) its velocity.  out is

@@ -115,6 +115,12 @@ _SIGS = {
                                                  ctypes.c_double, ctypes.POINTER(ctypes.c_double),
                                                  ctypes.POINTER(ctypes.c_double),
                                                  ctypes.POINTER(ctypes.c_double), _VP]),
+    "gsqg_spectral_diagnostics_noalloc": (ctypes.c_int, [_VP, _VP, ctypes.c_double,
+                                                         ctypes.POINTER(ctypes.c_double),
+                                                         ctypes.c_int, ctypes.c_double,
+                                                         ctypes.POINTER(ctypes.c_double),
+                                                         ctypes.POINTER(ctypes.c_double),
+                                                         ctypes.POINTER(ctypes.c_double), _VP]),
     "gsqg_physical_diagnostics": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(ctypes.c_double), _VP]),
     "gsqg_cfl_extrema": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(ctypes.c_double), _VP]),
     # include/gsqg_closure.h

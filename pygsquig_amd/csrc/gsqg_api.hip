@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1104,6 +1105,95 @@ extern "C" int gsqg_spectral_diagnostics(gsqg_plan* p, const void* X, double alp
   for (int b = 0; b < n_bins; ++b) bin_sums[b] = h[b];
   for (int s = 0; s < nshell; ++s) shell_sums[s] = h[n_bins + s];
   for (int s = 0; s < 3; ++s) sums[s] = h[n_bins + nshell + s];
+  return GSQG_OK;
+}
+
+// ---------------------------------------------- allocation-free diagnostics
+// gsqg_spectral_diagnostics_noalloc runs the kernels of
+// gsqg_spectral_diagnostics but keeps its scratch -- the per-block partial
+// sums, the reduced slots and the device copy of the bin edges -- in the
+// file-scope device array `scratch` instead of plan memory allocated on first
+// use.  The array is sized for n_bins <= kNaMaxBins at every supported N
+// (n <= 8192, so n/2 + 1 <= 4097 shells): 512 * 4356 + 4356 + 257 doubles,
+// 17.9 MB, held by each GPU the library runs on until the process ends.
+// Host code never passes `scratch` itself (in host code its name is a
+// stand-in, not a device address): it passes the address that
+// hipGetSymbolAddress returns for the current device, kept per device.
+static constexpr int kNaMaxBins = 256;
+static constexpr int kNaMaxShells = 8192 / 2 + 1;
+static constexpr int kNaStride = kNaMaxBins + kNaMaxShells + 3;
+static constexpr int kNaBlocks = 512;  // diag_blocks(), checked at run time
+static constexpr size_t kNaDoubles =
+    (size_t)kNaBlocks * kNaStride + (size_t)kNaStride + (size_t)(kNaMaxBins + 1);
+static __device__ __align__(256) double scratch[kNaDoubles];
+
+static constexpr int kNaMaxDevices = 64;
+static std::mutex g_na_mu;
+static double* g_na_ptr[kNaMaxDevices] = {};
+
+// Device address of `scratch` on `dev`, looked up once per device with that
+// device current (each GPU holds its own copy at its own address).
+static int noalloc_scratch(int dev, double** out) {
+  if (dev < 0 || dev >= kNaMaxDevices)
+    return fail(GSQG_ERR_UNSUPPORTED, "device index " + std::to_string(dev) + " out of range");
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (!g_na_ptr[dev]) {
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != dev) HIPCHK(hipSetDevice(dev));
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != dev) HIPCHK(hipSetDevice(cur));
+    if (e != hipSuccess || !ptr)
+      return fail(GSQG_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                    hipGetErrorString(e));
+    g_na_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_na_ptr[dev];
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_spectral_diagnostics_noalloc(gsqg_plan* p, const void* X, double alpha,
+                                                 const double* edges, int n_bins, double dk,
+                                                 double* bin_sums, double* shell_sums,
+                                                 double* sums, void* stream) {
+  if (!p || !X || !edges || !bin_sums || !shell_sums || !sums)
+    return fail(GSQG_ERR_ARG, "null argument");
+  if (n_bins < 1 || n_bins > kNaMaxBins)
+    return fail(GSQG_ERR_ARG, "n_bins must be in [1, " + std::to_string(kNaMaxBins) +
+                                  "] for the allocation-free diagnostics, got " +
+                                  std::to_string(n_bins));
+  if (!(edges[0] == 0.0) || !(edges[1] > 0.0) || !(dk > 0.0))
+    return fail(GSQG_ERR_ARG, "edges must start at 0 and increase; dk must be positive");
+  const int N = p->N;
+  const int nshell = N / 2 + 1;
+  const int stride = n_bins + nshell + 3;
+  if (nshell > kNaMaxShells || diag_blocks() != kNaBlocks)
+    return fail(GSQG_ERR_UNSUPPORTED, "allocation-free diagnostics: n = " + std::to_string(N) +
+                                          " exceeds the scratch array");
+  double* base = nullptr;
+  int rc;
+  if ((rc = noalloc_scratch(p->device, &base))) return rc;
+  // Every slot read below is written earlier in the same call (each block
+  // writes all `stride` slots of its partials, the reduction all of `out`),
+  // so nothing is cleared; calls that overlap on one GPU share the array and
+  // race.
+  double* part = base;
+  double* dout = base + (size_t)kNaBlocks * kNaStride;
+  double* dedges = dout + kNaStride;
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(hipMemcpyAsync(dedges, edges, (size_t)(n_bins + 1) * sizeof(double),
+                        hipMemcpyHostToDevice, st));
+  HIPCHK(launch_spec_diag((const double2*)X, p->k1d, dedges, N, n_bins, nshell, dk, alpha, part,
+                          stride, st));
+  HIPCHK(launch_diag_reduce(part, kNaBlocks, stride, stride, dout, st));
+  HIPCHK(hipMemcpyAsync(bin_sums, dout, (size_t)n_bins * sizeof(double), hipMemcpyDeviceToHost,
+                        st));
+  HIPCHK(hipMemcpyAsync(shell_sums, dout + n_bins, (size_t)nshell * sizeof(double),
+                        hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(sums, dout + n_bins + nshell, 3 * sizeof(double), hipMemcpyDeviceToHost,
+                        st));
+  HIPCHK(hipStreamSynchronize(st));
   return GSQG_OK;
 }
 

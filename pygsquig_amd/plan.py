@@ -209,17 +209,21 @@ class Plan:
                                               ctypes.c_uint64(key & 0xFFFFFFFFFFFFFFFF),
                                               _ptr(out), _stream()), "gsqg_ring_forcing")
 
-    def spectral_diagnostics(self, X: torch.Tensor, alpha: float, edges: np.ndarray, dk: float):
-        """(bin_sums, shell_sums, sums) of a full spectrum (gsqg_spectral_diagnostics)."""
+    def spectral_diagnostics(self, X: torch.Tensor, alpha: float, edges: np.ndarray, dk: float,
+                             noalloc: bool = False):
+        """(bin_sums, shell_sums, sums) of a full spectrum (gsqg_spectral_diagnostics;
+        with ``noalloc`` gsqg_spectral_diagnostics_noalloc, which allocates no
+        device memory and takes at most 256 bins)."""
         edges = np.ascontiguousarray(edges, dtype=np.float64)
         nb = edges.shape[0] - 1
         bins = np.zeros(nb)
         shells = np.zeros(self.grid.N // 2 + 1)
         sums = np.zeros(3)
         dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
-        _lib.check(self.lib.gsqg_spectral_diagnostics(self.handle, _ptr(X), float(alpha), dp(edges),
-                                                      nb, float(dk), dp(bins), dp(shells), dp(sums),
-                                                      _stream()), "gsqg_spectral_diagnostics")
+        name = "gsqg_spectral_diagnostics_noalloc" if noalloc else "gsqg_spectral_diagnostics"
+        _lib.check(getattr(self.lib, name)(self.handle, _ptr(X), float(alpha), dp(edges), nb,
+                                           float(dk), dp(bins), dp(shells), dp(sums), _stream()),
+                   name)
         return bins, shells, sums
 
     def physical_diagnostics(self, X: torch.Tensor):
