@@ -251,6 +251,7 @@ def main():
             "shapes": list(tuned[0]),
             "ms_per_pair": {f"{c},{r}": round(ms, 4) for (c, r), ms in tuned[1].items()}},
         "shapes": list(plan.shape()[:2]),
+        "shape_radix": [plan.shape_radix(k) for k in plan.shape()[:2]],
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
@@ -308,8 +309,11 @@ def bench_single(args):
                     "dt=1e-4")
     warm = min(args.warmup, 5) if args.config == "large" else args.warmup
     tuned = None
-    if args.config == "plain" and not args.no_autotune:  # N=8192 has one tile shape
-        tuned = solver.autotune(dt)
+    if not args.no_autotune:  # shapes picked by timing every pair (untimed here)
+        if args.config == "plain":
+            tuned = solver.autotune(dt)
+        else:  # N=8192: radix 8 and radix 16, a few steps per pair
+            tuned = solver.autotune(dt, forcing=f, damping=d, n_steps=4, repeats=1)
     capture()  # replay graphs captured before warm-up and timing
     st = run(st, warm)
     st, sec = _timed(lambda: run(st, steps), steps)
@@ -341,6 +345,7 @@ def bench_single(args):
         "autotune": None if tuned is None else {
             "shapes": list(tuned[0]),
             "ms_per_pair": {f"{c},{r}": round(ms, 4) for (c, r), ms in tuned[1].items()}},
+        "shape_radix": [plan.shape_radix(k) for k in plan.shape()[:2]],
         "kernels": kernels,
         "cpu_baseline": None,
         "note": "cpu_baseline is reported for the default configuration only"}))

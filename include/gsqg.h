@@ -176,11 +176,16 @@ int gsqg_ring_forcing(gsqg_plan* plan, const void* theta_hat_dev, uint64_t key, 
 int gsqg_profile_advance(gsqg_plan* plan, double dt, int n_steps, int key_mode, uint64_t key,
                          int64_t step0, double* class_ms, int* class_count, void* stream);
 
-/* Tile-shape autotuner (no counterpart in the reference, whose FFTs are
- * XLA's).  The column and row kernels are compiled in several tile shapes
- * (n_variants = 3 for n <= 512: the default 128-thread tile, a tile twice
- * as tall, one half as tall; 2 for n = 1024: the default and the taller
- * tile; 1 above).  gsqg_autotune times
+/* Tile-shape and radix autotuner (no counterpart in the reference, whose
+ * FFTs are XLA's).  The column and row kernels are compiled in several
+ * shapes, indexed 0 .. n_variants-1 with 0 the default: radix-8 schedules
+ * (8 elements per lane) in the default 128-thread tile, a tile twice as
+ * tall and one half as tall, and a radix-16 schedule (16 elements per lane,
+ * half the lanes per transform, fewer exchange passes).  n_variants = 3 for
+ * n <= 32 (radix-8 tiles), 4 for 64 <= n <= 512 (the three tiles and
+ * radix 16), 3 for n = 1024 (default and taller radix-8 tile, radix 16), 2
+ * above (radix 8 and radix 16); gsqg_plan_shape_radix reports the radix of
+ * a shape index.  gsqg_autotune times
  * gsqg_advance over n_steps steps (graphs on, explicit key 0) for every
  * (column, row) shape pair, best of `repeats` runs each, keeps the fastest
  * pair in the plan and returns it in best[2]; ms_table (n_variants^2 host
@@ -188,13 +193,16 @@ int gsqg_profile_advance(gsqg_plan* plan, double dt, int n_steps, int key_mode, 
  * The plan's state (theta, ring side state, scalars) is saved into a
  * temporary device buffer and restored afterwards; synchronises stream.
  * gsqg_plan_set_shape / gsqg_plan_shape set and read the pair directly;
- * changing the pair synchronises the device and drops captured graphs.  Every pair gives bit-identical states except
- * with ring forcing, whose <theta f> partial sums are then added in another
- * order (differences at the rounding level). */
+ * changing the pair synchronises the device and drops captured graphs.
+ * Pairs of one radix give bit-identical states except with ring forcing,
+ * whose <theta f> partial sums are then added in another order; radix-16
+ * transforms round differently from radix-8 ones.  Both differences are at
+ * the rounding level. */
 int gsqg_autotune(gsqg_plan* plan, double dt, int n_steps, int repeats, int* best,
                   double* ms_table, void* stream);
 int gsqg_plan_set_shape(gsqg_plan* plan, int col_shape, int row_shape);
 int gsqg_plan_shape(const gsqg_plan* plan, int* col_shape, int* row_shape, int* n_variants);
+int gsqg_plan_shape_radix(const gsqg_plan* plan, int shape, int* radix);
 
 /* Non-finite flag raised by the last steps (synchronises stream). */
 int gsqg_nonfinite(gsqg_plan* plan, int* flag, int clear, void* stream);

@@ -105,6 +105,7 @@ _SIGS = {
                                      ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
                                      _VP]),
     "gsqg_plan_set_shape": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int]),
+    "gsqg_plan_shape_radix": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "gsqg_plan_shape": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int),
                                        ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "gsqg_nonfinite": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int), ctypes.c_int, _VP]),

@@ -21,6 +21,7 @@ hipError_t launch_rk(int N, int job, const RKArgs& a, int ny, hipStream_t s, con
                      int var);
 int ck_num_blocks(int N, int var);
 int num_shape_variants(int N);
+int shape_radix(int N, int var);
 hipError_t launch_full_to_half(const double2* X, double2* H, int N, double* norms, double* norm0,
                                hipStream_t s);
 int full_to_half_num_blocks(int N);
@@ -220,6 +221,15 @@ extern "C" int gsqg_plan_create(const gsqg_plan_desc* d, gsqg_plan** out) {
   ph.combine = d->combine;
   ph.hv_dc_zero = d->hv_dc_zero;
   ph.kappa = 0.0;
+  // The register-tight kernels recompute kx from its index; the table must
+  // follow the same formula, fftfreq(n, L/n) * 2 pi as k / (dx n) * 2 * pi.
+  ph.dxn = (d->L / N) * N;
+  for (int i = 0; i < N; ++i) {
+    const int ki = i < N / 2 ? i : i - N;
+    if (((double)ki / ph.dxn) * 2.0 * 3.141592653589793 != d->k1d[i])
+      return fail(GSQG_ERR_ARG, "k1d[" + std::to_string(i) +
+                                    "] is not k / (L/n * n) * 2 * pi (core/grid.py:94)");
+  }
   p->n_scalars = d->n_scalars;
   for (int s = 0; s < d->n_scalars; ++s) p->kappa[s] = d->kappa ? d->kappa[s] : 0.0;
 
@@ -938,6 +948,15 @@ extern "C" int gsqg_plan_shape(const gsqg_plan* p, int* col_shape, int* row_shap
   if (col_shape) *col_shape = p->ck_var;
   if (row_shape) *row_shape = p->rk_var;
   if (n_variants) *n_variants = p->n_var;
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_plan_shape_radix(const gsqg_plan* p, int shape, int* radix) {
+  if (!p || !radix) return fail(GSQG_ERR_ARG, "null argument");
+  if (shape < 0 || shape >= p->n_var)
+    return fail(GSQG_ERR_ARG, "shape variant out of range [0, " + std::to_string(p->n_var) +
+                                  ") for N = " + std::to_string(p->N));
+  *radix = shape_radix(p->N, shape);
   return GSQG_OK;
 }
 

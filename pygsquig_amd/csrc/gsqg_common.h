@@ -115,11 +115,83 @@ __device__ __forceinline__ void dft8(cd (&v)[8]) {
   v[7] = csub(e3, t3);
 }
 
+// In-register 16-point DFT, natural order in and out: two 8-point DFTs of
+// the even and odd elements, combined with W16^k = exp(DIR i pi k / 8).
+template <int DIR>
+__device__ __forceinline__ void dft16(cd (&v)[16]) {
+  const double c1 = 0.92387953251128675613;  // cos(pi/8)
+  const double s1 = 0.38268343236508977173;  // sin(pi/8)
+  const double r2 = 0.70710678118654752440;  // 1/sqrt(2)
+  cd e[8], o[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    e[k] = v[2 * k];
+    o[k] = v[2 * k + 1];
+  }
+  dft8<DIR>(e);
+  dft8<DIR>(o);
+  // W16^k for k = 1..7 with DIR < 0 (the inverse uses the conjugates):
+  // (c1,-s1) (r2,-r2) (s1,-c1) (0,-1) (-s1,-c1) (-r2,-r2) (-c1,-s1)
+  const double sg = DIR < 0 ? -1.0 : 1.0;
+  cd t[8];
+  t[0] = o[0];
+  t[1] = cmul(o[1], cmk(c1, sg * s1));
+  t[2] = cmul(o[2], cmk(r2, sg * r2));
+  t[3] = cmul(o[3], cmk(s1, sg * c1));
+  t[4] = rot90<DIR>(o[4]);
+  t[5] = cmul(o[5], cmk(-s1, sg * c1));
+  t[6] = cmul(o[6], cmk(-r2, sg * r2));
+  t[7] = cmul(o[7], cmk(-c1, sg * s1));
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    v[k] = cadd(e[k], t[k]);
+    v[k + 8] = csub(e[k], t[k]);
+  }
+}
+
 // Twiddle tables: tw[T + k] = exp(-2 pi i k / T) for every power of two T.
 template <int DIR>
 __device__ __forceinline__ cd twiddle(const cd* __restrict__ tw, int T, int k) {
   cd w = tw[T + k];
   return DIR < 0 ? w : cconj(w);
+}
+
+// x[r] *= w^r, r = 1..7, from w, w^2, w^4 with the products of the twiddle
+// sources' pass(): w^3 = w w^2, w^5 = w w^4, w^6 = w^2 w^4, w^7 = w^3 w^4.
+__device__ __forceinline__ void apply_w124(cd w1, cd w2, cd w4, cd (&x)[8]) {
+  x[1] = cmul(x[1], w1);
+  x[2] = cmul(x[2], w2);
+  x[4] = cmul(x[4], w4);
+  const cd w3 = cmul(w1, w2);
+  x[3] = cmul(x[3], w3);
+  x[5] = cmul(x[5], cmul(w1, w4));
+  x[6] = cmul(x[6], cmul(w2, w4));
+  x[7] = cmul(x[7], cmul(w3, w4));
+}
+
+// x[r] *= w^r, r = 1..15, from w, w^2, w^4, w^8 with the products of
+// pass16(): w^3 = w w^2, w^5 = w w^4, w^6 = w^2 w^4, w^7 = w^3 w^4 and
+// w^(r+8) = w^r w^8.
+__device__ __forceinline__ void apply_w1248(cd w1, cd w2, cd w4, cd w8, cd (&x)[16]) {
+  x[8] = cmul(x[8], w8);
+  x[1] = cmul(x[1], w1);
+  x[9] = cmul(x[9], cmul(w1, w8));
+  x[2] = cmul(x[2], w2);
+  x[10] = cmul(x[10], cmul(w2, w8));
+  x[4] = cmul(x[4], w4);
+  x[12] = cmul(x[12], cmul(w4, w8));
+  const cd w3 = cmul(w1, w2);
+  x[3] = cmul(x[3], w3);
+  x[11] = cmul(x[11], cmul(w3, w8));
+  const cd w5 = cmul(w1, w4);
+  x[5] = cmul(x[5], w5);
+  x[13] = cmul(x[13], cmul(w5, w8));
+  const cd w6 = cmul(w2, w4);
+  x[6] = cmul(x[6], w6);
+  x[14] = cmul(x[14], cmul(w6, w8));
+  const cd w7 = cmul(w3, w4);
+  x[7] = cmul(x[7], w7);
+  x[15] = cmul(x[15], cmul(w7, w8));
 }
 
 // Twiddle sources.  Radix-8 pass p >= 1 (Ns = 8^p) needs w, w^2, w^4 with
@@ -139,6 +211,35 @@ struct TwGlobal {
     w[5] = cmul(w[1], w[4]);
     w[6] = cmul(w[2], w[4]);
     w[7] = cmul(w[3], w[4]);
+  }
+  // x[r] *= W^(r jm) with the same products as pass(), formed where each is
+  // used: three twiddles live instead of seven (one transform at a time).
+  template <int DIR>
+  __device__ __forceinline__ void apply(int Ns, int jm, cd (&x)[8]) const {
+    const cd w1 = twiddle<DIR>(tw, Ns * 8, jm);
+    const cd w2 = twiddle<DIR>(tw, Ns * 8, 2 * jm);
+    const cd w4 = twiddle<DIR>(tw, Ns * 8, 4 * jm);
+    apply_w124(w1, w2, w4, x);
+  }
+  // Radix-16 pass: w[r] = W^(r jm), W = exp(DIR 2 pi i / (16 Ns)), from
+  // four table reads (w, w^2, w^4, w^8) and eleven products.
+  template <int DIR>
+  __device__ __forceinline__ void pass16(int Ns, int jm, cd (&w)[16]) const {
+    w[1] = twiddle<DIR>(tw, Ns * 16, jm);
+    w[2] = twiddle<DIR>(tw, Ns * 16, 2 * jm);
+    w[4] = twiddle<DIR>(tw, Ns * 16, 4 * jm);
+    w[8] = twiddle<DIR>(tw, Ns * 16, 8 * jm);
+    w[3] = cmul(w[1], w[2]);
+    w[5] = cmul(w[1], w[4]);
+    w[6] = cmul(w[2], w[4]);
+    w[7] = cmul(w[3], w[4]);
+#pragma unroll
+    for (int r = 1; r < 8; ++r) w[r + 8] = cmul(w[r], w[8]);
+  }
+  template <int DIR>
+  __device__ __forceinline__ void apply16(int Ns, int jm, cd (&x)[16]) const {
+    apply_w1248(twiddle<DIR>(tw, Ns * 16, jm), twiddle<DIR>(tw, Ns * 16, 2 * jm),
+                twiddle<DIR>(tw, Ns * 16, 4 * jm), twiddle<DIR>(tw, Ns * 16, 8 * jm), x);
   }
   template <int DIR>
   __device__ __forceinline__ cd fin2(int, int L, int j) const {
@@ -207,6 +308,32 @@ struct TwOct {
     w[7] = cmul(w[3], w[4]);
   }
   template <int DIR>
+  __device__ __forceinline__ void apply(int Ns, int jm, cd (&x)[8]) const {
+    const int s = LT / (8 * Ns);
+    apply_w124(dir<DIR>(o.fwd(jm * s)), dir<DIR>(o.fwd(2 * jm * s)), dir<DIR>(o.fwd(4 * jm * s)),
+               x);
+  }
+  template <int DIR>
+  __device__ __forceinline__ void pass16(int Ns, int jm, cd (&w)[16]) const {
+    const int s = LT / (16 * Ns);
+    w[1] = dir<DIR>(o.fwd(jm * s));
+    w[2] = dir<DIR>(o.fwd(2 * jm * s));
+    w[4] = dir<DIR>(o.fwd(4 * jm * s));
+    w[8] = dir<DIR>(o.fwd(8 * jm * s));
+    w[3] = cmul(w[1], w[2]);
+    w[5] = cmul(w[1], w[4]);
+    w[6] = cmul(w[2], w[4]);
+    w[7] = cmul(w[3], w[4]);
+#pragma unroll
+    for (int r = 1; r < 8; ++r) w[r + 8] = cmul(w[r], w[8]);
+  }
+  template <int DIR>
+  __device__ __forceinline__ void apply16(int Ns, int jm, cd (&x)[16]) const {
+    const int s = LT / (16 * Ns);
+    apply_w1248(dir<DIR>(o.fwd(jm * s)), dir<DIR>(o.fwd(2 * jm * s)), dir<DIR>(o.fwd(4 * jm * s)),
+                dir<DIR>(o.fwd(8 * jm * s)), x);
+  }
+  template <int DIR>
   __device__ __forceinline__ cd fin2(int, int, int j) const {
     return dir<DIR>(o.fwd(j * (LT / L)));
   }
@@ -226,7 +353,7 @@ struct TwOct {
 // share one set of barriers and one set of twiddle reads.  All lanes of the
 // block must call it together (it contains block barriers).
 template <int L, int DIR, int G, class TW>
-__device__ __forceinline__ void fft_core(cd (&v)[G][8], cd* buf, int gs, int t, const TW& tws) {
+__device__ __forceinline__ void fft_core8(cd (&v)[G][8], cd* buf, int gs, int t, const TW& tws) {
   static_assert(L >= 8 && (L & (L - 1)) == 0, "power of two >= 8");
   constexpr int Q = L / 8;
   constexpr int LOG = __builtin_ctz(L);
@@ -244,12 +371,16 @@ __device__ __forceinline__ void fft_core(cd (&v)[G][8], cd* buf, int gs, int t, 
     }
     if (Ns > 1) {
       const int jm = t % Ns;
-      cd w[8];
-      tws.template pass<DIR>(Ns, jm, w);
+      if constexpr (G == 1) {
+        tws.template apply<DIR>(Ns, jm, v[0]);
+      } else {
+        cd w[8];
+        tws.template pass<DIR>(Ns, jm, w);
 #pragma unroll
-      for (int g = 0; g < G; ++g) {
+        for (int g = 0; g < G; ++g) {
 #pragma unroll
-        for (int r = 1; r < 8; ++r) v[g][r] = cmul(v[g][r], w[r]);
+          for (int r = 1; r < 8; ++r) v[g][r] = cmul(v[g][r], w[r]);
+        }
       }
     }
 #pragma unroll
@@ -305,11 +436,128 @@ __device__ __forceinline__ void fft_core(cd (&v)[G][8], cd* buf, int gs, int t, 
   }
 }
 
+// The same over L/16 lanes with 16 elements per lane (lane t holds elements
+// t + m*L/16, m = 0..15): radix-16 Stockham passes, then a radix-2, -4 or -8
+// pass for the remaining factor of L, whose butterflies each lane does in
+// place on registers q + s*16/F.
+template <int L, int DIR, int G, class TW>
+__device__ __forceinline__ void fft_core16(cd (&v)[G][16], cd* buf, int gs, int t, const TW& tws) {
+  static_assert(L >= 16 && (L & (L - 1)) == 0, "power of two >= 16");
+  constexpr int Q = L / 16;
+  constexpr int LOG = __builtin_ctz(L);
+  constexpr int P16 = LOG / 4;
+  constexpr int REM = LOG % 4;
+  int Ns = 1;
+#pragma unroll
+  for (int p = 0; p < P16; ++p) {
+    if (p > 0) {
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+#pragma unroll
+        for (int m = 0; m < 16; ++m) v[g][m] = buf[g * gs + lpad(t + m * Q)];
+      }
+    }
+    if (Ns > 1) {
+      const int jm = t % Ns;
+      if constexpr (G == 1) {
+        tws.template apply16<DIR>(Ns, jm, v[0]);
+      } else {
+        cd w[16];
+        tws.template pass16<DIR>(Ns, jm, w);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+#pragma unroll
+          for (int r = 1; r < 16; ++r) v[g][r] = cmul(v[g][r], w[r]);
+        }
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) dft16<DIR>(v[g]);
+    const bool last = (p == P16 - 1) && (REM == 0);
+    if (!last) {
+      __syncthreads();
+      const int base = (t / Ns) * Ns * 16 + (t % Ns);
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) buf[g * gs + lpad(base + r * Ns)] = v[g][r];
+      }
+      __syncthreads();
+    }
+    Ns *= 16;
+  }
+  if (REM != 0 && P16 > 0) {
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+#pragma unroll
+      for (int m = 0; m < 16; ++m) v[g][m] = buf[g * gs + lpad(t + m * Q)];
+    }
+  }
+  if (REM == 1) {  // radix 2, Ns = L/2: pairs (q, q + 8)
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const cd w = tws.template fin2<DIR>(0, L, t + q * Q);
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        cd a = v[g][q], b = cmul(v[g][q + 8], w);
+        dft2<DIR>(a, b);
+        v[g][q] = a;
+        v[g][q + 8] = b;
+      }
+    }
+  } else if (REM == 2) {  // radix 4, Ns = L/4: (q, q + 4, q + 8, q + 12)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      cd w1, w2, w3;
+      tws.template fin4<DIR>(0, L, t + q * Q, w1, w2, w3);
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        cd a0 = v[g][q], a1 = cmul(v[g][q + 4], w1), a2 = cmul(v[g][q + 8], w2),
+           a3 = cmul(v[g][q + 12], w3);
+        dft4<DIR>(a0, a1, a2, a3);
+        v[g][q] = a0;
+        v[g][q + 4] = a1;
+        v[g][q + 8] = a2;
+        v[g][q + 12] = a3;
+      }
+    }
+  } else if (REM == 3) {  // radix 8, Ns = L/8: (q, q + 2, ..., q + 14)
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      cd w[8];
+      tws.template pass<DIR>(L / 8, t + q * Q, w);
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        cd a[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) a[s] = v[g][q + 2 * s];
+        if constexpr (G == 1)
+          tws.template apply<DIR>(L / 8, t + q * Q, a);
+        else
+          for (int s = 1; s < 8; ++s) a[s] = cmul(a[s], w[s]);
+        dft8<DIR>(a);
+#pragma unroll
+        for (int s = 0; s < 8; ++s) v[g][q + 2 * s] = a[s];
+      }
+    }
+  }
+}
+
+// Dispatch on the elements per lane (8 or 16), deduced from v.
+template <int L, int DIR, int G, class TW>
+__device__ __forceinline__ void fft_core(cd (&v)[G][8], cd* buf, int gs, int t, const TW& tws) {
+  fft_core8<L, DIR, G>(v, buf, gs, t, tws);
+}
+template <int L, int DIR, int G, class TW>
+__device__ __forceinline__ void fft_core(cd (&v)[G][16], cd* buf, int gs, int t, const TW& tws) {
+  fft_core16<L, DIR, G>(v, buf, gs, t, tws);
+}
+
 // One transform with twiddles from the global table; `buf` is its LDS
 // region of Padded<L> elements.
-template <int L, int DIR>
-__device__ __forceinline__ void fft_lanes(cd (&v)[8], cd* buf, int t, const cd* __restrict__ tw) {
-  fft_core<L, DIR, 1>(*reinterpret_cast<cd(*)[1][8]>(&v), buf, 0, t, TwGlobal{tw});
+template <int L, int DIR, int R>
+__device__ __forceinline__ void fft_lanes(cd (&v)[R], cd* buf, int t, const cd* __restrict__ tw) {
+  fft_core<L, DIR, 1>(*reinterpret_cast<cd(*)[1][R]>(&v), buf, 0, t, TwGlobal{tw});
 }
 
 }  // namespace gsqg

@@ -50,6 +50,7 @@ struct Phys {
   int combine;      // 0 dt*(k1+2k2+2k3+k4)/6, 1 (dt/6)*(...)
   int hv_dc_zero;   // 1: hyperviscosity zeroed at k=0 (reference gSQGSolver)
   double kappa;     // scalar diffusivity
+  double dxn;       // (L / N) * N, the divisor of the wavenumber table k1d
 };
 
 // One mode of the anti-Hermitian ring side state with its wavenumbers and

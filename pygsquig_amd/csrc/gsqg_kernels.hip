@@ -25,13 +25,18 @@
 namespace gsqg {
 
 // ------------------------------------------------------------------ shapes
-// Shape variant V, chosen per plan by the autotuner (gsqg_autotune): 0 is
-// the default tile of 128 threads; 1 doubles and 2 halves the rows (column
-// kernel) or x-rows (row kernel) of a block.  Variant 1 is compiled for
-// N <= 1024, variant 2 for N <= 512.
+// Shape variant V, chosen per plan by the autotuner (gsqg_autotune).
+// Variants 0-2 run the radix-8 schedule (8 elements per lane): 0 is the
+// default tile of 128 threads; 1 doubles and 2 halves the rows (column
+// kernel) or x-rows (row kernel) of a block.  Variant 3 runs the radix-16
+// schedule (16 elements per lane, half the lanes per transform): fewer LDS
+// exchange passes and, at large N, blocks whose register budget holds a
+// whole row without spilling.  Which variants each N compiles is listed in
+// variant_id() below.
 template <int N, int V = 0>
 struct CKShape {
-  static constexpr int Q = N / 8;  // lanes per row
+  static constexpr int R = V == 3 ? 16 : 8;  // elements per lane
+  static constexpr int Q = N / R;            // lanes per row
   static constexpr int R00 = (128 / Q) > 1 ? (128 / Q) : 1;
   static constexpr int R0 = V == 1 ? 2 * R00 : (V == 2 && R00 > 1 ? R00 / 2 : R00);
   static constexpr int RPW = R0 < (N / 2) ? R0 : (N / 2);  // rows per block
@@ -47,9 +52,15 @@ struct CKShape {
 template <int N, int V = 0>
 struct RKShape {
   static constexpr int M = N / 2;
-  static constexpr int Q = M / 8;  // lanes per x-row
+  static constexpr int R = V == 3 ? 16 : 8;  // elements per lane
+  static constexpr int Q = M / R;            // lanes per x-row
   static constexpr int W00 = (128 / Q) > 1 ? (128 / Q) : 1;
-  static constexpr int W0 = V == 1 ? 2 * W00 : (V == 2 && W00 > 1 ? W00 / 2 : W00);
+  // Radix-16 tiles at 256 lanes per x-line take two x-lines, so that each
+  // row of a tile is 32 contiguous bytes of the ky-major fields.
+  static constexpr int W0 = V == 1   ? 2 * W00
+                            : V == 2 ? (W00 > 1 ? W00 / 2 : W00)
+                            : V == 3 ? (Q >= 256 ? 2 : W00)
+                                     : W00;
   static constexpr int WX = W0 < N ? W0 : N;  // x-rows per block
   static constexpr int NT = WX * Q;
   static constexpr int PADM = Padded<M>::value;
@@ -72,6 +83,13 @@ struct RKFields {
 };
 
 // ------------------------------------------------------- per-mode physics
+// |k|^2 as the reference forms it, KX**2 + KY**2 (core/grid.py:96): each
+// square and the sum rounded separately.  Never contracted to an fma, so
+// every kernel and stage that needs |k|^2 of a mode gets the same value.
+__device__ __forceinline__ double k2_of(double kx, double ky) {
+  return __dadd_rn(__dmul_rn(kx, kx), __dmul_rn(ky, ky));
+}
+
 __device__ __forceinline__ double hv_factor(double k2, int p) {
   if (p == 2) return k2;
   double s = k2 * k2;
@@ -308,47 +326,67 @@ __device__ __forceinline__ cd r_from_z(const cd* buf, int r, int N) {
 // Per-lane wavenumbers and flags of the column kernel, loaded at kernel
 // start: kx and the raw flag byte of elements t + m*Q of row j, the row's ky
 // and k1d[N/2] (row 0 holds the ky = N/2 modes too).
+//
+// With KC (the register-tight shapes) kx is not kept per element but
+// recomputed from its index the way the host builds the table,
+// k / (dx N) * 2 * pi with dxn = dx N: the same three correctly rounded
+// operations, so the values are bit-identical (plan creation checks the
+// table against this formula).  The flag bytes are packed four to a word.
+constexpr double kPi = 3.141592653589793;
+template <int N, int R, bool KC>
 struct RowCtx {
-  double kx[8];
-  unsigned fb[8];
-  double ky, kyn;
+  double kxs[KC ? 1 : R];
+  uint32_t fbw[(R + 3) / 4];
+  double ky, kyn, dxn;
+  int t;
+  __device__ __forceinline__ double kx(int m) const {
+    if constexpr (KC) {
+      const int i = t + m * (N / R);
+      const int ki = i < N / 2 ? i : i - N;
+      return ((double)ki / dxn) * 2.0 * kPi;
+    } else {
+      return kxs[m];
+    }
+  }
+  __device__ __forceinline__ unsigned fb(int m) const { return (fbw[m >> 2] >> (8 * (m & 3))) & 0xffu; }
 };
 
 // Mode of element m (not a row-0 self-conjugate element): wavenumbers,
 // flags, indices and the Nyquist lines its multipliers vanish on.  Row 0 is
-// in R layout: elements m < 4 are (r, 0), elements m >= 4 are (r - N/2, N/2),
-// so both use kx and the flag byte of element m & 3.
+// in R layout: with R elements per lane, elements m < R/2 are (r, 0) and
+// elements m >= R/2 are (r - N/2, N/2), so both use kx and the flag byte of
+// element m mod R/2.
 struct MInfo {
   double kx, ky;
   unsigned fl;
   int ii, jy;
   bool kxnyq, kynyq;
 };
-template <int N>
-__device__ __forceinline__ MInfo minfo(const RowCtx& c, int m, int t, int j, bool row0) {
-  constexpr int Q = N / 8;
+template <int N, int R, bool KC>
+__device__ __forceinline__ MInfo minfo(const RowCtx<N, R, KC>& c, int m, int t, int j, bool row0) {
+  constexpr int Q = N / R;
   MInfo r;
   if (!row0) {
-    r.kx = c.kx[m];
+    r.kx = c.kx(m);
     r.ky = c.ky;
-    r.fl = c.fb[m] & 15u;
+    r.fl = c.fb(m) & 15u;
     r.ii = t + m * Q;
     r.jy = j;
     r.kxnyq = (r.ii == N / 2);
     r.kynyq = false;
   } else {
-    const int m4 = m & 3;
-    r.kx = c.kx[m4];
+    const int m4 = m & (R / 2 - 1);
+    r.kx = c.kx(m4);
     r.ii = t + m4 * Q;
     r.kxnyq = false;
-    if (m < 4) {
+    if (m < R / 2) {
       r.ky = c.ky;
-      r.fl = c.fb[m4] & 15u;
+      r.fl = c.fb(m4) & 15u;
       r.jy = 0;
       r.kynyq = false;
     } else {
       r.ky = c.kyn;
-      r.fl = c.fb[m4] >> 4;
+      r.fl = c.fb(m4) >> 4;
       r.jy = N / 2;
       r.kynyq = true;
     }
@@ -382,7 +420,7 @@ __device__ __forceinline__ ModeOut row0_selfc_epilogue(int E, int r, const ModeI
       ring_parts(key, r, jy, N, ph.amp, PF, QF);
       sF = cscale(PF, sc);
     }
-    oo[h] = mode_epilogue<JOB == CK_SCALAR>(E, u, kxr * kxr + ky * ky, (r == 0 && jy == 0), fl,
+    oo[h] = mode_epilogue<JOB == CK_SCALAR>(E, u, k2_of(kxr, ky), (r == 0 && jy == 0), fl,
                                             ph, dt, FORCED, sF);
   }
   ModeOut o;
@@ -396,6 +434,7 @@ __device__ __forceinline__ ModeOut row0_selfc_epilogue(int E, int r, const ModeI
 template <int N, int JOB, int E, bool FORCED, int V>
 __global__ void __launch_bounds__((CKShape<N, V>::NT)) ck_kernel(CKArgs a) {
   using S = CKShape<N, V>;
+  constexpr int R = S::R;
   constexpr int Q = S::Q;
   constexpr int PADL = S::PADL;
   constexpr int NT = S::NT;
@@ -419,29 +458,29 @@ __global__ void __launch_bounds__((CKShape<N, V>::NT)) ck_kernel(CKArgs a) {
   const bool row0 = (j == 0);
   const bool blk0 = (blockIdx.x == 0);  // block-uniform: row 0 lives in block 0
 
-  cd v[8];
+  cd v[R];
 
   // ---------------------------------------------------- API transforms
   if (JOB == CK_FWD || JOB == CK_INV) {
     const cd* src = (JOB == CK_FWD) ? a.J : a.th;
     cd* dst = (JOB == CK_FWD) ? a.th : a.F[0];
 #pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = src[ro + t + m * Q];
+    for (int m = 0; m < R; ++m) v[m] = src[ro + t + m * Q];
     if (JOB == CK_INV) {
       if (blk0) {  // row 0: R layout -> packed Z
         __syncthreads();
         if (row0) {
 #pragma unroll
-          for (int m = 0; m < 8; ++m) buf[lpad(t + m * Q)] = v[m];
+          for (int m = 0; m < R; ++m) buf[lpad(t + m * Q)] = v[m];
         }
         __syncthreads();
         if (row0) {
 #pragma unroll
-          for (int m = 0; m < 8; ++m) v[m] = z_from_r(buf, t + m * Q, N);
+          for (int m = 0; m < R; ++m) v[m] = z_from_r(buf, t + m * Q, N);
         }
       }
 #pragma unroll
-      for (int m = 0; m < 8; ++m) v[m] = cscale(v[m], ph.inv_n2);
+      for (int m = 0; m < R; ++m) v[m] = cscale(v[m], ph.inv_n2);
       fft_lanes<N, +1>(v, buf, t, tw);
     } else {
       fft_lanes<N, -1>(v, buf, t, tw);
@@ -449,17 +488,17 @@ __global__ void __launch_bounds__((CKShape<N, V>::NT)) ck_kernel(CKArgs a) {
         __syncthreads();
         if (row0) {
 #pragma unroll
-          for (int m = 0; m < 8; ++m) buf[lpad(t + m * Q)] = v[m];
+          for (int m = 0; m < R; ++m) buf[lpad(t + m * Q)] = v[m];
         }
         __syncthreads();
         if (row0) {
 #pragma unroll
-          for (int m = 0; m < 8; ++m) v[m] = r_from_z(buf, t + m * Q, N);
+          for (int m = 0; m < R; ++m) v[m] = r_from_z(buf, t + m * Q, N);
         }
       }
     }
 #pragma unroll
-    for (int m = 0; m < 8; ++m) dst[ro + t + m * Q] = v[m];
+    for (int m = 0; m < R; ++m) dst[ro + t + m * Q] = v[m];
     return;
   }
 
@@ -475,17 +514,18 @@ __global__ void __launch_bounds__((CKShape<N, V>::NT)) ck_kernel(CKArgs a) {
     key = ctrl_key(ctrl, step);
     if (E > 0) sc = ctrl->scale[E];
   }
-  cd y[8];
-  // LEAN (1024-lane blocks, 128 registers per lane): the epilogue's per-mode
-  // state is loaded where it is used instead of up front, which avoids
-  // spilling at the cost of loads queued behind earlier stores.
-  constexpr bool LEAN = NT > 512;
-  constexpr int NPRE = LEAN ? 1 : 8;
+  cd y[R];
+  // LEAN (1024-lane blocks with 128 registers per lane, or 16 elements per
+  // lane): the epilogue's per-mode state is loaded where it is used instead
+  // of up front, which avoids spilling at the cost of loads queued behind
+  // earlier stores.
+  constexpr bool LEAN = NT > 512 || R > 8;
+  constexpr int NPRE = LEAN ? 1 : R;
   cd thv[NPRE], accv[NPRE], ysv[NPRE];
   const bool need_ys = (ph.scheme == 0) && (E >= 3);
   if (E > 0) {
 #pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = a.J[ro + t + m * Q];
+    for (int m = 0; m < R; ++m) v[m] = a.J[ro + t + m * Q];
     if (!LEAN) {
 #pragma unroll
       for (int m = 0; m < NPRE; ++m) thv[m] = a.th[ro + t + m * Q];
@@ -496,13 +536,17 @@ __global__ void __launch_bounds__((CKShape<N, V>::NT)) ck_kernel(CKArgs a) {
     }
   } else {
 #pragma unroll
-    for (int m = 0; m < 8; ++m) y[m] = a.th[ro + t + m * Q];
+    for (int m = 0; m < R; ++m) y[m] = a.th[ro + t + m * Q];
   }
-  RowCtx ctx;
+  RowCtx<N, R, LEAN> ctx;
+  ctx.t = t;
+  ctx.dxn = ph.dxn;
 #pragma unroll
-  for (int m = 0; m < 8; ++m) {
-    ctx.kx[m] = k1d[t + m * Q];
-    ctx.fb[m] = a.flags[(size_t)j * N + t + m * Q];
+  for (int m = 0; m < (R + 3) / 4; ++m) ctx.fbw[m] = 0u;
+#pragma unroll
+  for (int m = 0; m < R; ++m) {
+    if constexpr (!LEAN) ctx.kxs[m] = k1d[t + m * Q];
+    ctx.fbw[m >> 2] |= (uint32_t)a.flags[(size_t)j * N + t + m * Q] << (8 * (m & 3));
   }
   ctx.ky = k1d[j];
   ctx.kyn = k1d[N / 2];
@@ -521,35 +565,35 @@ __global__ void __launch_bounds__((CKShape<N, V>::NT)) ck_kernel(CKArgs a) {
     at0.dc = ap->dc;
     A0 = a.a_val[ent0];
   }
-  if (TWL) OctTab<N>::fill(twl, tw, tid, NT);
+  if constexpr (TWL) OctTab<N>::fill(twl, tw, tid, NT);
   __syncthreads();
   const TwOct<N, N> twS{{twl}};
   const TwGlobal twG{tw};
 
   // --------------------------------------------------------- epilogue
   if (E > 0) {
-    if (TWL)
-      fft_core<N, -1, 1>(*reinterpret_cast<cd(*)[1][8]>(&v), buf, 0, t, twS);
+    if constexpr (TWL)
+      fft_core<N, -1, 1>(*reinterpret_cast<cd(*)[1][R]>(&v), buf, 0, t, twS);
     else
-      fft_core<N, -1, 1>(*reinterpret_cast<cd(*)[1][8]>(&v), buf, 0, t, twG);
+      fft_core<N, -1, 1>(*reinterpret_cast<cd(*)[1][R]>(&v), buf, 0, t, twG);
 
     if (blk0) {  // row 0: forward-transformed Jacobian, packed Z -> R layout
       __syncthreads();
       if (row0) {
 #pragma unroll
-        for (int m = 0; m < 8; ++m) buf[lpad(t + m * Q)] = v[m];
+        for (int m = 0; m < R; ++m) buf[lpad(t + m * Q)] = v[m];
       }
       __syncthreads();
       if (row0) {
 #pragma unroll
-        for (int m = 0; m < 8; ++m) v[m] = r_from_z(buf, t + m * Q, N);
+        for (int m = 0; m < R; ++m) v[m] = r_from_z(buf, t + m * Q, N);
       }
     }
 
     // With every table preloaded, no global load follows these stores, so
     // they are issued as soon as each mode is done.
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
+    for (int m = 0; m < R; ++m) {
       ModeIn in;
       in.J = v[m];
       if (LEAN) {
@@ -562,20 +606,20 @@ __global__ void __launch_bounds__((CKShape<N, V>::NT)) ck_kernel(CKArgs a) {
         in.acc = accv[m];
         in.ys = ysv[m];
       }
-      const bool selfc = row0 && t == 0 && (m == 0 || m == 4);
+      const bool selfc = row0 && t == 0 && (m == 0 || m == R / 2);
       ModeOut o;
       if (!selfc) {
-        const MInfo mi = minfo<N>(ctx, m, t, j, row0);
+        const MInfo mi = minfo<N, R>(ctx, m, t, j, row0);
         cd sF = cmk(0.0, 0.0);
         if (FORCED && (mi.fl & FLAG_RING)) {
           cd PF, QF;
           ring_parts(key, mi.ii, mi.jy, N, ph.amp, PF, QF);
           sF = cscale(PF, sc);
         }
-        o = mode_epilogue<JOB == CK_SCALAR>(E, in, mi.kx * mi.kx + mi.ky * mi.ky, false, mi.fl,
+        o = mode_epilogue<JOB == CK_SCALAR>(E, in, k2_of(mi.kx, mi.ky), false, mi.fl,
                                             ph, a.dt, FORCED, sF);
       } else {
-        o = row0_selfc_epilogue<JOB, FORCED>(E, m * Q, in, ctx.kx[m], ctx.fb[m], ctx.ky, ctx.kyn,
+        o = row0_selfc_epilogue<JOB, FORCED>(E, m * Q, in, ctx.kx(m), ctx.fb(m), ctx.ky, ctx.kyn,
                                              ph, a.dt, key, sc);
       }
       const int i = t + m * Q;
@@ -600,7 +644,7 @@ __global__ void __launch_bounds__((CKShape<N, V>::NT)) ck_kernel(CKArgs a) {
           at.dc = ap->dc;
           A = a.a_val[ent];
         }
-        const double k2 = at.kx * at.kx + at.ky * at.ky;
+        const double k2 = k2_of(at.kx, at.ky);
         const bool dc = at.dc != 0;
         cd PF, QF;
         ring_parts(key, at.ii, at.jy, N, ph.amp, PF, QF);
@@ -641,7 +685,7 @@ __global__ void __launch_bounds__((CKShape<N, V>::NT)) ck_kernel(CKArgs a) {
     if (E == 4 && JOB == CK_THETA) {
       bool bad = false;
 #pragma unroll
-      for (int m = 0; m < 8; ++m) bad |= !(isfinite(y[m].x) && isfinite(y[m].y));
+      for (int m = 0; m < R; ++m) bad |= !(isfinite(y[m].x) && isfinite(y[m].y));
       if (bad) a.ctrl->nonfinite = 1.0;
     }
   }
@@ -650,7 +694,7 @@ __global__ void __launch_bounds__((CKShape<N, V>::NT)) ck_kernel(CKArgs a) {
   if (!a.prologue) return;
   if (a.ystage) {  // the stage state itself, for the closure's features
 #pragma unroll
-    for (int m = 0; m < 8; ++m) a.ystage[ro + t + m * Q] = y[m];
+    for (int m = 0; m < R; ++m) a.ystage[ro + t + m * Q] = y[m];
   }
 
   // <theta f> partials of the prologue state (ring_forcing.py:153-155 by
@@ -659,10 +703,10 @@ __global__ void __launch_bounds__((CKShape<N, V>::NT)) ck_kernel(CKArgs a) {
     const uint64_t rkey = ctrl_key(ctrl, a.rate_next_key ? step + 1 : step);
     double accr = 0.0;
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const bool selfc = row0 && t == 0 && (m == 0 || m == 4);
+    for (int m = 0; m < R; ++m) {
+      const bool selfc = row0 && t == 0 && (m == 0 || m == R / 2);
       if (!selfc) {
-        const MInfo mi = minfo<N>(ctx, m, t, j, row0);
+        const MInfo mi = minfo<N, R>(ctx, m, t, j, row0);
         if (mi.fl & FLAG_RING) {
           cd PF, QF;
           ring_parts(rkey, mi.ii, mi.jy, N, ph.amp, PF, QF);
@@ -672,7 +716,7 @@ __global__ void __launch_bounds__((CKShape<N, V>::NT)) ck_kernel(CKArgs a) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           const int jy = h ? N / 2 : 0;
-          const unsigned fl = h ? (ctx.fb[m] >> 4) : (ctx.fb[m] & 15u);
+          const unsigned fl = h ? (ctx.fb(m) >> 4) : (ctx.fb(m) & 15u);
           if (fl & FLAG_RING) {
             cd PF, QF;
             ring_parts(rkey, m * Q, jy, N, ph.amp, PF, QF);
@@ -701,13 +745,27 @@ __global__ void __launch_bounds__((CKShape<N, V>::NT)) ck_kernel(CKArgs a) {
   const int akind = a.member_alpha ? alpha_kind_of(alpha) : ph.alpha_kind;
 #pragma unroll
   for (int g0 = 0; g0 < NF; g0 += G) {
-    cd w[G][8];
+    cd w[G][R];
     const bool needK = (JOB == CK_VEL) || (JOB == CK_THETA && g0 + G > 2);
+    // Register-tight shapes scale y by |k|^-alpha in place before the first
+    // group that needs it (the groups without it come first), so that no
+    // per-mode K stays live across a transform.
+    constexpr bool KIN = LEAN;
+    if constexpr (KIN) {
+      if (needK && g0 == (JOB == CK_VEL ? 0 : 2)) {
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const bool selfc = row0 && t == 0 && (m == 0 || m == 4);  // their fields vanish
-      const MInfo mi = minfo<N>(ctx, m, t, j, row0);
-      const double K = needK ? inv_frac(mi.kx * mi.kx + mi.ky * mi.ky, alpha, akind) : 0.0;
+        for (int m = 0; m < R; ++m) {
+          const MInfo mi = minfo<N, R>(ctx, m, t, j, row0);
+          y[m] = cscale(y[m], inv_frac(k2_of(mi.kx, mi.ky), alpha, akind));
+        }
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      const bool selfc = row0 && t == 0 && (m == 0 || m == R / 2);  // their fields vanish
+      const MInfo mi = minfo<N, R>(ctx, m, t, j, row0);
+      const double K = KIN ? 1.0
+                           : needK ? inv_frac(k2_of(mi.kx, mi.ky), alpha, akind) : 0.0;
 #pragma unroll
       for (int g = 0; g < G; ++g)
         w[g][m] = selfc ? cmk(0.0, 0.0)
@@ -720,7 +778,7 @@ __global__ void __launch_bounds__((CKShape<N, V>::NT)) ck_kernel(CKArgs a) {
 #pragma unroll
         for (int g = 0; g < G; ++g) {
 #pragma unroll
-          for (int m = 0; m < 8; ++m) buf[g * RG + lpad(t + m * Q)] = w[g][m];
+          for (int m = 0; m < R; ++m) buf[g * RG + lpad(t + m * Q)] = w[g][m];
         }
       }
       __syncthreads();
@@ -728,11 +786,11 @@ __global__ void __launch_bounds__((CKShape<N, V>::NT)) ck_kernel(CKArgs a) {
 #pragma unroll
         for (int g = 0; g < G; ++g) {
 #pragma unroll
-          for (int m = 0; m < 8; ++m) w[g][m] = z_from_r(buf + g * RG, t + m * Q, N);
+          for (int m = 0; m < R; ++m) w[g][m] = z_from_r(buf + g * RG, t + m * Q, N);
         }
       }
     }
-    if (TWL)
+    if constexpr (TWL)
       fft_core<N, +1, G>(w, buf, RG, t, twS);
     else
       fft_core<N, +1, G>(w, buf, RG, t, twG);
@@ -740,7 +798,7 @@ __global__ void __launch_bounds__((CKShape<N, V>::NT)) ck_kernel(CKArgs a) {
     for (int g = 0; g < G; ++g) {
       cd* dst = a.F[g0 + g] + fo + (size_t)j * N;
 #pragma unroll
-      for (int m = 0; m < 8; ++m) dst[t + m * Q] = w[g][m];
+      for (int m = 0; m < R; ++m) dst[t + m * Q] = w[g][m];
     }
   }
 }
@@ -762,14 +820,14 @@ __device__ __forceinline__ int rk_tile(int b) {
 // through scratch).
 template <int N, int V, int G>
 __device__ __forceinline__ void rk_issue(const cd* const (&src)[G], int x0, int tid,
-                                         vd2 (&tmp)[G * 8]) {
+                                         vd2 (&tmp)[G * RKShape<N, V>::R]) {
   using S = RKShape<N, V>;
-  constexpr int WX = S::WX, NT = S::NT;
+  constexpr int WX = S::WX, NT = S::NT, R = S::R;
 #pragma unroll
-  for (int e = 0; e < G * 8; ++e) {
-    const int l = tid + (e & 7) * NT;
+  for (int e = 0; e < G * R; ++e) {
+    const int l = tid + (e % R) * NT;
     const int jr = l / WX, bb = l - jr * WX;
-    tmp[e] = *reinterpret_cast<const vd2*>(src[e >> 3] + (size_t)jr * N + x0 + bb);
+    tmp[e] = *reinterpret_cast<const vd2*>(src[e / R] + (size_t)jr * N + x0 + bb);
   }
 }
 
@@ -790,20 +848,21 @@ struct PreTw {
 // with pre(k) = e^{-2 pi i k/N}.  Ends without a barrier: the next LDS write
 // (inside the transform) is preceded by one.
 template <int N, int V, int G, class PT>
-__device__ __forceinline__ void rk_commit(const vd2 (&tmp)[G * 8], cd* lds, int tid, int b, int t,
-                                          const PT& pre, cd (&v)[G][8]) {
+__device__ __forceinline__ void rk_commit(const vd2 (&tmp)[G * RKShape<N, V>::R], cd* lds, int tid,
+                                          int b, int t, const PT& pre,
+                                          cd (&v)[G][RKShape<N, V>::R]) {
   using S = RKShape<N, V>;
-  constexpr int M = S::M, WX = S::WX, NT = S::NT, Q = S::Q, RG = S::REGION;
+  constexpr int M = S::M, WX = S::WX, NT = S::NT, Q = S::Q, RG = S::REGION, R = S::R;
   __syncthreads();
 #pragma unroll
-  for (int e = 0; e < G * 8; ++e) {
-    const int l = tid + (e & 7) * NT;
+  for (int e = 0; e < G * R; ++e) {
+    const int l = tid + (e % R) * NT;
     const int jr = l / WX, bb = l - jr * WX;
-    *reinterpret_cast<vd2*>(lds + (e >> 3) * RG + bb * S::RSTR + lpad(jr)) = tmp[e];
+    *reinterpret_cast<vd2*>(lds + (e / R) * RG + bb * S::RSTR + lpad(jr)) = tmp[e];
   }
   __syncthreads();
 #pragma unroll
-  for (int m = 0; m < 8; ++m) {
+  for (int m = 0; m < R; ++m) {
     const int k = t + m * Q;
     const cd w = cconj(pre(k));
 #pragma unroll
@@ -822,18 +881,19 @@ __device__ __forceinline__ void rk_commit(const vd2 (&tmp)[G * 8], cd* lds, int 
 
 // Forward half-length FFT result in v -> post-twist -> store X(k), packed
 // slot 0 = X(0) + i X(M), at dst[k*N + x0 + b] (coalesced along x).
-template <int N, int V, class PT>
-__device__ __forceinline__ void rk_store_r2c(cd (&v)[8], cd* dst, int x0, cd* lds, int tid, int b,
+template <int N, int V, int R, class PT>
+__device__ __forceinline__ void rk_store_r2c(cd (&v)[R], cd* dst, int x0, cd* lds, int tid, int b,
                                              int t, const PT& pre) {
   using S = RKShape<N, V>;
+  static_assert(R == S::R, "elements per lane of the shape");
   constexpr int M = S::M, WX = S::WX, NT = S::NT, Q = S::Q;
   cd* buf = lds + b * S::RSTR;
   __syncthreads();
 #pragma unroll
-  for (int m = 0; m < 8; ++m) buf[lpad(t + m * Q)] = v[m];
+  for (int m = 0; m < R; ++m) buf[lpad(t + m * Q)] = v[m];
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < 8; ++r) {
+  for (int r = 0; r < R; ++r) {
     const int l = tid + r * NT;
     const int jr = l / WX, bb = l - jr * WX;
     const cd* bf = lds + bb * S::RSTR;
@@ -854,7 +914,7 @@ __device__ __forceinline__ void rk_store_r2c(cd (&v)[8], cd* dst, int x0, cd* ld
 template <int N, int JOB, int V>
 __global__ void __launch_bounds__((RKShape<N, V>::NT)) rk_kernel(RKArgs a) {
   using S = RKShape<N, V>;
-  constexpr int M = S::M, Q = S::Q, WX = S::WX, NT = S::NT, RG = S::REGION;
+  constexpr int M = S::M, Q = S::Q, WX = S::WX, NT = S::NT, RG = S::REGION, R = S::R;
   constexpr int NF = RKFields<JOB>::value;
   constexpr int G = S::FP < NF ? S::FP : NF;  // fields per lockstep group
   constexpr bool THETA = (JOB == RK_THETA || JOB == RK_THETA_CL);
@@ -898,13 +958,13 @@ __global__ void __launch_bounds__((RKShape<N, V>::NT)) rk_kernel(RKArgs a) {
 
   // Loads first (they share one in-order completion counter with stores):
   // the first tile group, the frozen velocity, the LDS tables.
-  vd2 tmp[G * 8];
-  cd v[8];
-  cd uu[8], vv[8];
+  vd2 tmp[G * R];
+  cd v[R];
+  cd uu[R], vv[R];
   if (JOB == RK_R2C) {
     const cd* src = reinterpret_cast<const cd*>(a.phys_in + fo + (size_t)x * N);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = src[t + m * Q];
+    for (int m = 0; m < R; ++m) v[m] = src[t + m * Q];
   } else {
     const cd* src[G];
 #pragma unroll
@@ -916,32 +976,32 @@ __global__ void __launch_bounds__((RKShape<N, V>::NT)) rk_kernel(RKArgs a) {
   }
   // closure flux of this x-line as packed real pairs (y = 2k, 2k + 1)
   typedef float vf2 __attribute__((vector_size(8)));
-  vf2 tz[JOB == RK_THETA_CL ? 8 : 1];
+  vf2 tz[JOB == RK_THETA_CL ? R : 1];
   if (JOB == RK_THETA_CL) {
     const vf2* tp = reinterpret_cast<const vf2*>(a.tau + (size_t)x * N);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) tz[m] = tp[t + m * Q];
+    for (int m = 0; m < R; ++m) tz[m] = tp[t + m * Q];
   }
   if (JOB == RK_SCALAR) {
     const cd* up = reinterpret_cast<const cd*>(a.uphys + (size_t)x * N);
     const cd* vp = reinterpret_cast<const cd*>(a.vphys + (size_t)x * N);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
+    for (int m = 0; m < R; ++m) {
       uu[m] = up[t + m * Q];
       vv[m] = vp[t + m * Q];
     }
   }
-  if (TWL) OctTab<N>::fill(twl, tw, tid, NT);
+  if constexpr (TWL) OctTab<N>::fill(twl, tw, tid, NT);
   const PreTw<N, TWL> pre{TWL ? twl : tw};
   const TwOct<M, N> twS{{twl}};
   const TwGlobal twG{tw};
 
   if (JOB == RK_R2C) {
     __syncthreads();
-    if (TWL)
-      fft_core<M, -1, 1>(*reinterpret_cast<cd(*)[1][8]>(&v), buf, 0, t, twS);
+    if constexpr (TWL)
+      fft_core<M, -1, 1>(*reinterpret_cast<cd(*)[1][R]>(&v), buf, 0, t, twS);
     else
-      fft_core<M, -1, 1>(*reinterpret_cast<cd(*)[1][8]>(&v), buf, 0, t, twG);
+      fft_core<M, -1, 1>(*reinterpret_cast<cd(*)[1][R]>(&v), buf, 0, t, twG);
     rk_store_r2c<N, V>(v, a.Jout + fo, x0, lds, tid, b, t, pre);
     return;
   }
@@ -949,12 +1009,12 @@ __global__ void __launch_bounds__((RKShape<N, V>::NT)) rk_kernel(RKArgs a) {
   // Theta: fields in the order theta_x, u, theta_y, v (F[0], F[2], F[1],
   // F[3]), J = u * theta_x + v * theta_y on packed real pairs
   // z(k) = (r(2k), r(2k+1)); scalars: s_x, s_y against the frozen u, v.
-  cd jz[8], hold[8];
+  cd jz[R], hold[R];
 #pragma unroll
   for (int g0 = 0; g0 < NF; g0 += G) {
-    cd w[G][8];
+    cd w[G][R];
     rk_commit<N, V, G>(tmp, lds, tid, b, t, pre, w);
-    if (TWL)
+    if constexpr (TWL)
       fft_core<M, +1, G>(w, buf, RG, t, twS);
     else
       fft_core<M, +1, G>(w, buf, RG, t, twG);
@@ -975,34 +1035,34 @@ __global__ void __launch_bounds__((RKShape<N, V>::NT)) rk_kernel(RKArgs a) {
         double* dstp = (JOB == RK_C2R) ? a.phys_out : (s == 0 ? a.uout : a.vout);
         cd* dst = reinterpret_cast<cd*>(dstp + fo + (size_t)x * N);
 #pragma unroll
-        for (int m = 0; m < 8; ++m) dst[t + m * Q] = w[g][m];
+        for (int m = 0; m < R; ++m) dst[t + m * Q] = w[g][m];
       } else if (THETA) {
         if ((s & 1) == 0) {
 #pragma unroll
-          for (int m = 0; m < 8; ++m) hold[m] = w[g][m];
+          for (int m = 0; m < R; ++m) hold[m] = w[g][m];
         } else {
           double* side = (s == 1) ? a.uout : a.vout;
           if (side) {
             cd* dst = reinterpret_cast<cd*>(side + (size_t)x * N);
 #pragma unroll
-            for (int m = 0; m < 8; ++m) dst[t + m * Q] = w[g][m];
+            for (int m = 0; m < R; ++m) dst[t + m * Q] = w[g][m];
           }
           if (s == 1) {
 #pragma unroll
-            for (int m = 0; m < 8; ++m)
+            for (int m = 0; m < R; ++m)
               jz[m] = cmk(w[g][m].x * hold[m].x, w[g][m].y * hold[m].y);
           } else {
 #pragma unroll
-            for (int m = 0; m < 8; ++m)
+            for (int m = 0; m < R; ++m)
               jz[m] = cmk(jz[m].x + w[g][m].x * hold[m].x, jz[m].y + w[g][m].y * hold[m].y);
           }
         }
       } else if (s == 0) {
 #pragma unroll
-        for (int m = 0; m < 8; ++m) jz[m] = cmk(uu[m].x * w[g][m].x, uu[m].y * w[g][m].y);
+        for (int m = 0; m < R; ++m) jz[m] = cmk(uu[m].x * w[g][m].x, uu[m].y * w[g][m].y);
       } else {
 #pragma unroll
-        for (int m = 0; m < 8; ++m)
+        for (int m = 0; m < R; ++m)
           jz[m] = cmk(jz[m].x + vv[m].x * w[g][m].x, jz[m].y + vv[m].y * w[g][m].y);
       }
     }
@@ -1010,16 +1070,16 @@ __global__ void __launch_bounds__((RKShape<N, V>::NT)) rk_kernel(RKArgs a) {
   if (JOB == RK_C2R || JOB == RK_VEL) return;
   if (JOB == RK_THETA_CL) {
 #pragma unroll
-    for (int m = 0; m < 8; ++m)
+    for (int m = 0; m < R; ++m)
       jz[m] = cmk(jz[m].x + (double)tz[m][0], jz[m].y + (double)tz[m][1]);
   }
 #pragma unroll
-  for (int m = 0; m < 8; ++m) v[m] = jz[m];
+  for (int m = 0; m < R; ++m) v[m] = jz[m];
   __syncthreads();
-  if (TWL)
-    fft_core<M, -1, 1>(*reinterpret_cast<cd(*)[1][8]>(&v), buf, 0, t, twS);
+  if constexpr (TWL)
+    fft_core<M, -1, 1>(*reinterpret_cast<cd(*)[1][R]>(&v), buf, 0, t, twS);
   else
-    fft_core<M, -1, 1>(*reinterpret_cast<cd(*)[1][8]>(&v), buf, 0, t, twG);
+    fft_core<M, -1, 1>(*reinterpret_cast<cd(*)[1][R]>(&v), buf, 0, t, twG);
   rk_store_r2c<N, V>(v, a.Jout + fo, x0, lds, tid, b, t, pre);
 }
 
@@ -1205,7 +1265,7 @@ __global__ void __launch_bounds__(256) spec_diag_kernel(const cd* __restrict__ X
     const int i = (int)(e / N), j = (int)(e - (long long)i * N);
     const cd x = X[e];
     const double kx = k1d[i], ky = k1d[j];
-    const double k2 = __dadd_rn(__dmul_rn(kx, kx), __dmul_rn(ky, ky));
+    const double k2 = k2_of(kx, ky);
     const double a2 = __dadd_rn(__dmul_rn(x.x, x.x), __dmul_rn(x.y, x.y));
     const bool dc = (i == 0 && j == 0);
     const double kmag = sqrt(k2);
@@ -1355,7 +1415,7 @@ __global__ void a_rhs_kernel(Phys ph, const double* k1d, const uint8_t* flags, c
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
     const int i = modes[e].x, jy = modes[e].y;
     const double kx = k1d[i], ky = k1d[jy];
-    const double k2 = kx * kx + ky * ky;
+    const double k2 = k2_of(kx, ky);
     const unsigned fl = mode_flags(flags, N, i, jy);
     cd PF, QF;
     ring_parts(key, i, jy, N, ph.amp, PF, QF);
@@ -1477,36 +1537,58 @@ static hipError_t rk_by_job_v(int job, const RKArgs& a, int ny, hipStream_t s, c
   return hipErrorInvalidValue;
 }
 
-// Number of shape variants compiled for grid size N (at N = 1024 the
-// default column block is one row, so there is no smaller variant).
+// Shape variants compiled for grid size N, in the autotuner's index order;
+// index 0 is the default.  Radix-8 tiles 0-2 where a block of 128 threads
+// holds several rows (at N = 1024 the default column block is one row, so
+// there is no smaller tile), and the radix-16 variant 3 from N = 64 up.
 template <int N>
-constexpr int n_variants() { return N <= 512 ? 3 : (N == 1024 ? 2 : 1); }
+constexpr int n_variants() {
+  return N <= 32 ? 3 : N <= 512 ? 4 : N == 1024 ? 3 : 2;
+}
+template <int N>
+constexpr int variant_id(int k) {
+  if (N >= 2048) return k == 0 ? 0 : 3;
+  if (N == 1024) return k == 2 ? 3 : k;
+  return k;  // 0, 1, 2 (and 3 from N = 64)
+}
 
-template <int N>
+template <int N, int K = 0>
 hipError_t ck_by_job(int job, const CKArgs& a, int ny, hipStream_t s, const LaunchEv* ev, int var) {
-  if constexpr (n_variants<N>() > 1)
-    if (var == 1) return ck_by_job_v<N, 1>(job, a, ny, s, ev);
-  if constexpr (n_variants<N>() > 2)
-    if (var == 2) return ck_by_job_v<N, 2>(job, a, ny, s, ev);
-  return ck_by_job_v<N, 0>(job, a, ny, s, ev);
+  if constexpr (K < n_variants<N>()) {
+    if (var == K || K + 1 == n_variants<N>()) return ck_by_job_v<N, variant_id<N>(K)>(job, a, ny, s, ev);
+    return ck_by_job<N, K + 1>(job, a, ny, s, ev, var);
+  }
+  return hipErrorInvalidValue;
 }
 
-template <int N>
+template <int N, int K = 0>
 hipError_t rk_by_job(int job, const RKArgs& a, int ny, hipStream_t s, const LaunchEv* ev, int var) {
-  if constexpr (n_variants<N>() > 1)
-    if (var == 1) return rk_by_job_v<N, 1>(job, a, ny, s, ev);
-  if constexpr (n_variants<N>() > 2)
-    if (var == 2) return rk_by_job_v<N, 2>(job, a, ny, s, ev);
-  return rk_by_job_v<N, 0>(job, a, ny, s, ev);
+  if constexpr (K < n_variants<N>()) {
+    if (var == K || K + 1 == n_variants<N>()) return rk_by_job_v<N, variant_id<N>(K)>(job, a, ny, s, ev);
+    return rk_by_job<N, K + 1>(job, a, ny, s, ev, var);
+  }
+  return hipErrorInvalidValue;
 }
 
-template <int N>
+template <int N, int K = 0>
 int ck_blocks_n(int var) {
-  if constexpr (n_variants<N>() > 1)
-    if (var == 1) return (N / 2) / CKShape<N, 1>::RPW;
-  if constexpr (n_variants<N>() > 2)
-    if (var == 2) return (N / 2) / CKShape<N, 2>::RPW;
-  return (N / 2) / CKShape<N, 0>::RPW;
+  if constexpr (K < n_variants<N>()) {
+    if (var == K || K + 1 == n_variants<N>()) {
+      using S = CKShape<N, variant_id<N>(K)>;
+      return (N / 2) / S::RPW;
+    }
+    return ck_blocks_n<N, K + 1>(var);
+  }
+  return 0;
+}
+
+template <int N, int K = 0>
+int radix_n(int var) {
+  if constexpr (K < n_variants<N>()) {
+    if (var == K || K + 1 == n_variants<N>()) return CKShape<N, variant_id<N>(K)>::R;
+    return radix_n<N, K + 1>(var);
+  }
+  return 0;
 }
 
 #define GSQG_FOR_EACH_N(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) X(8192)
@@ -1551,6 +1633,14 @@ hipError_t launch_rk(int N, int job, const RKArgs& a, int ny, hipStream_t s, con
 int ck_num_blocks(int N, int var) {
   switch (N) {
 #define X(N_) GSQG_CASE(N_, ck_blocks_n<N_>(var))
+    GSQG_FOR_EACH_N(X)
+#undef X
+  }
+  return 0;
+}
+int shape_radix(int N, int var) {
+  switch (N) {
+#define X(N_) GSQG_CASE(N_, radix_n<N_>(var))
     GSQG_FOR_EACH_N(X)
 #undef X
   }

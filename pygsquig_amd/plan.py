@@ -261,6 +261,13 @@ class Plan:
                                             ctypes.byref(n)), "gsqg_plan_shape")
         return c.value, r.value, n.value
 
+    def shape_radix(self, shape: int) -> int:
+        """Elements per lane (the FFT radix, 8 or 16) of shape index ``shape``."""
+        r = ctypes.c_int(0)
+        _lib.check(self.lib.gsqg_plan_shape_radix(self.handle, int(shape), ctypes.byref(r)),
+                   "gsqg_plan_shape_radix")
+        return r.value
+
     def set_shape(self, col: int, row: int) -> None:
         _lib.check(self.lib.gsqg_plan_set_shape(self.handle, int(col), int(row)),
                    "gsqg_plan_set_shape")
