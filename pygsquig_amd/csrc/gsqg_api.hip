@@ -531,7 +531,12 @@ static int launch_scalar_rk(gsqg_plan* p, const double* u, const double* v, hipS
   a.ctrl = p->ctrl;
   a.part = p->part;
   a.field_stride = (long long)p->hsz;
-  HIPCHK(rk(p, RK_SCALAR, a, p->n_scalars, st));
+  // Large grids: one block per x-line tile transforms every scalar against
+  // the velocity it loads once (grids of N >= 2048 have blocks to spare);
+  // below, each scalar is a y-slice of the launch, for parallelism.
+  const bool loop = p->N >= 2048 && p->n_scalars > 1;
+  a.n_loop = loop ? p->n_scalars : 1;
+  HIPCHK(rk(p, RK_SCALAR, a, loop ? 1 : p->n_scalars, st));
   return GSQG_OK;
 }
 

@@ -119,6 +119,7 @@ struct RKArgs {
   double eps;
   long long field_stride;
   const float* tau;      // RK_THETA_CL: physical (N, N) flux added to J
+  int n_loop;            // RK_SCALAR, N >= 2048: scalars one block transforms in turn
 };
 
 }  // namespace gsqg

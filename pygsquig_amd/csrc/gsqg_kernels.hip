@@ -1009,31 +1009,52 @@ __global__ void __launch_bounds__((RKShape<N, V>::NT)) rk_kernel(RKArgs a) {
   // Theta: fields in the order theta_x, u, theta_y, v (F[0], F[2], F[1],
   // F[3]), J = u * theta_x + v * theta_y on packed real pairs
   // z(k) = (r(2k), r(2k+1)); scalars: s_x, s_y against the frozen u, v.
+  //
+  // With n_loop > 1 (scalars, N >= 2048) one block transforms every scalar
+  // of its x-lines in turn against the velocity it loaded once; the loads of
+  // the next scalar's first tile go out ahead of this scalar's stores.
+  constexpr bool SLOOP = JOB == RK_SCALAR && N >= 2048;
+  const int nloop = (SLOOP && a.n_loop > 1) ? a.n_loop : 1;
+  for (int sc = 0; sc < nloop; ++sc) {
+  const size_t fos = fo + (size_t)sc * (size_t)a.field_stride;
   cd jz[R], hold[R];
 #pragma unroll
   for (int g0 = 0; g0 < NF; g0 += G) {
     cd w[G][R];
     rk_commit<N, V, G>(tmp, lds, tid, b, t, pre, w);
+    // The next group's loads go out ahead of this group's stores: for theta
+    // as soon as this group's tiles are in registers, so that they travel
+    // during this group's transforms; for the scalars (whose velocity
+    // occupies registers) after them.
+    if constexpr (THETA) {
+      if (g0 + G < NF) {
+        const cd* src[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          const int s = g0 + G + g;
+          src[g] = a.F[(s & 1) ? 2 + (s >> 1) : (s >> 1)] + fos;
+        }
+        rk_issue<N, V, G>(src, x0, tid, tmp);
+      }
+    }
     if constexpr (TWL)
       fft_core<M, +1, G>(w, buf, RG, t, twS);
     else
       fft_core<M, +1, G>(w, buf, RG, t, twG);
-    if (g0 + G < NF) {  // next group's loads, ahead of this group's stores
-      const cd* src[G];
+    if constexpr (!THETA) {
+      if (g0 + G < NF) {
+        const cd* src[G];
 #pragma unroll
-      for (int g = 0; g < G; ++g) {
-        const int s = g0 + G + g;
-        const int q = THETA ? ((s & 1) ? 2 + (s >> 1) : (s >> 1)) : s;
-        src[g] = a.F[q] + fo;
+        for (int g = 0; g < G; ++g) src[g] = a.F[g0 + G + g] + fos;
+        rk_issue<N, V, G>(src, x0, tid, tmp);
       }
-      rk_issue<N, V, G>(src, x0, tid, tmp);
     }
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       const int s = g0 + g;
       if (JOB == RK_C2R || JOB == RK_VEL) {
         double* dstp = (JOB == RK_C2R) ? a.phys_out : (s == 0 ? a.uout : a.vout);
-        cd* dst = reinterpret_cast<cd*>(dstp + fo + (size_t)x * N);
+        cd* dst = reinterpret_cast<cd*>(dstp + fos + (size_t)x * N);
 #pragma unroll
         for (int m = 0; m < R; ++m) dst[t + m * Q] = w[g][m];
       } else if (THETA) {
@@ -1073,6 +1094,12 @@ __global__ void __launch_bounds__((RKShape<N, V>::NT)) rk_kernel(RKArgs a) {
     for (int m = 0; m < R; ++m)
       jz[m] = cmk(jz[m].x + (double)tz[m][0], jz[m].y + (double)tz[m][1]);
   }
+  if (SLOOP && sc + 1 < nloop) {  // the next scalar's first tile group
+    const cd* src[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) src[g] = a.F[g] + fos + (size_t)a.field_stride;
+    rk_issue<N, V, G>(src, x0, tid, tmp);
+  }
 #pragma unroll
   for (int m = 0; m < R; ++m) v[m] = jz[m];
   __syncthreads();
@@ -1080,7 +1107,8 @@ __global__ void __launch_bounds__((RKShape<N, V>::NT)) rk_kernel(RKArgs a) {
     fft_core<M, -1, 1>(*reinterpret_cast<cd(*)[1][R]>(&v), buf, 0, t, twS);
   else
     fft_core<M, -1, 1>(*reinterpret_cast<cd(*)[1][R]>(&v), buf, 0, t, twG);
-  rk_store_r2c<N, V>(v, a.Jout + fo, x0, lds, tid, b, t, pre);
+  rk_store_r2c<N, V>(v, a.Jout + fos, x0, lds, tid, b, t, pre);
+  }
 }
 
 #if defined(GSQG_INSTANTIATE_COMMON)
