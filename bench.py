@@ -27,7 +27,14 @@ set (each its own JSON line, same keys):
          lr=1e-3), then the closed N=512 run against the filtered DNS and the
          unclosed N=512 run: shell-spectrum error for k <= 170 and wall-clock
          per simulated time unit.  --nf/--nc/--snaps/--train-steps/--t-eval
-         shrink it for quick checks.
+         shrink it for quick checks;
+  closure-decay  the learned closure on decaying turbulence, in about a
+         minute: eight seeded k^-5/3 realizations at N=512 (alpha=1,
+         nu_p=1e-15, p=8, dt=1e-3), 12 snapshots each filtered to N=256,
+         training on six, then closed and unclosed N=256 runs from each
+         held-out realization against the filtered DNS over one time unit
+         (the gain the closure tests check, with wall-clocks; the coarse
+         runs take dt=2e-3, the DNS's CFL number on the coarser grid).
 """
 
 from __future__ import annotations
@@ -96,7 +103,8 @@ def main():
     ap.add_argument("--copy-only", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates PMC byte counters)")
     ap.add_argument("--config", default="forced",
-                    choices=["forced", "plain", "large", "batch", "closure", "cnn"])
+                    choices=["forced", "plain", "large", "batch", "closure", "closure-decay",
+                             "cnn"])
     # closure workload sizes
     ap.add_argument("--nf", type=int, default=4096)
     ap.add_argument("--nc", type=int, default=512)
@@ -105,8 +113,9 @@ def main():
     ap.add_argument("--train-steps", type=int, default=1500)
     ap.add_argument("--t-spin", type=float, default=0.5)
     ap.add_argument("--t-eval", type=float, default=10.0)
-    ap.add_argument("--dt-coarse", default="1e-4",
-                    help="coarse time step; a comma list evaluates each (the first is reported)")
+    ap.add_argument("--dt-coarse", default=None,
+                    help="coarse time step (closure: 1e-4, closure-decay: 2e-3); a comma list "
+                         "evaluates each (closure; the first is reported)")
     ap.add_argument("--nu", type=float, default=None,
                     help="hyperviscosity (default 1e-8 * (512 / nf)^8)")
     ap.add_argument("--coarse-seeds", default="43",
@@ -114,9 +123,10 @@ def main():
                          "measure the spread between forcing realizations")
     ap.add_argument("--no-match-forcing", dest="match_forcing", action="store_false",
                     help="coarse forcing amplitude sqrt(eps) instead of sqrt(eps) (nc/nf)^2")
-    ap.add_argument("--closure-every", default="8",
-                    help="closure evaluated every M steps (0: every RK4 stage); a comma list "
-                         "evaluates each (the first is reported)")
+    ap.add_argument("--closure-every", default=None,
+                    help="closure evaluated every M steps (0: every RK4 stage; closure: 8, "
+                         "closure-decay: 4,0,8); a comma list evaluates each (the first is "
+                         "reported)")
     args = ap.parse_args()
     if args.copy_only:
         import torch
@@ -137,6 +147,8 @@ def main():
         return bench_batch(args)
     if args.config == "closure":
         return bench_closure(args)
+    if args.config == "closure-decay":
+        return bench_closure_decay(args)
     if args.config == "cnn":
         return bench_cnn(args)
     if args.config in ("plain", "large"):
@@ -469,7 +481,7 @@ def bench_closure(args):
     dev = torch.device("cuda:0")
     L, alpha = 2 * np.pi, 1.0
     Nf, Nc = args.nf, args.nc
-    dt_list = [float(x) for x in str(args.dt_coarse).split(",")]
+    dt_list = [float(x) for x in str(args.dt_coarse or "1e-4").split(",")]
     dt_f, dt_c = 1e-4, dt_list[0]
     nu = 1e-8 * (512 / Nf) ** 8 if args.nu is None else args.nu
     kf, dk, eps, mu = 40.0, 2.0, 0.1, 0.1
@@ -551,7 +563,7 @@ def bench_closure(args):
     # threshold as the DNS (only the random phases differ).
     amp_c = np.sqrt(eps) * (Nc / Nf) ** 2 if args.match_forcing else None
     forcing_c = pg.RingForcing(kf=kf, dk=dk, epsilon=eps, seed=seeds[0], amplitude=amp_c)
-    every_list = [int(x) for x in str(args.closure_every).split(",")]
+    every_list = [int(x) for x in str(args.closure_every or "8").split(",")]
     closed_by = {m: C.ClosedSolver(gc, alpha, 0.0, 8, net=net, in_scale=in_scale,
                                    out_scale=1.0 / tgt_scale, scheme="if", every=m)
                  for m in every_list}
@@ -657,6 +669,119 @@ def bench_closure(args):
                      "tau_rms": rms_j},
         "spectra": {"k": ks.tolist(), "dns": E_dns[ks].tolist(), "closed": E_cl[ks].tolist(),
                     "unclosed": E_pl[ks].tolist()},
+        "cpu_baseline": None,
+        "note": "cpu_baseline is reported for the default configuration only",
+    }
+    print(json.dumps(out))
+
+
+def bench_closure_decay(args):
+    """Decaying DNS N=512 -> closed and unclosed N=256 runs on held-out
+    realizations: mean log(E / E_DNS) per shell band and wall-clock per
+    simulated time unit."""
+    import numpy as np
+    import torch
+
+    import pygsquig_amd as pg
+    from pygsquig_amd import closure as C
+    from pygsquig_amd import ics
+
+    dev = torch.device("cuda:0")
+    L, alpha, nu, dt = 2 * np.pi, 1.0, 1e-15, 1e-3
+    Nf, Nc = 512, 256
+    n_real, n_test, n_snap, gap, spin, n_eval = 8, 2, 12, 50, 2000, 1000
+    # the coarse grid's spacing is twice the DNS's: 2 dt is the DNS's CFL number
+    dt_c = float(str(args.dt_coarse).split(",")[0]) if args.dt_coarse else 2 * dt
+    every_list = [int(x) for x in str(args.closure_every or "4,0,8").split(",")]
+    t_start = time.perf_counter()
+
+    def log(msg):
+        print(f"[closure-decay {time.perf_counter() - t_start:6.1f}s] {msg}", flush=True)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return out, time.perf_counter() - t0
+
+    gf, gc = pg.make_grid(Nf, L, dev), pg.make_grid(Nc, L, dev)
+    dns = pg.gSQGSolver(gf, alpha, nu, 8, scheme="if")
+    sampler = C.SubgridSampler(gf, gc, alpha)
+    store = C.SnapshotStore(n_real * n_snap, Nc, dev)
+    ends = []
+    for r in range(n_real):
+        st = dns.initialize(torch.from_numpy(ics.kolmogorov_53(Nf, seed=50 + r)).to(dev))
+        st = dns.multistep(st, spin, dt)
+        for _ in range(n_snap):
+            st = dns.multistep(st, gap, dt)
+            _, th, u, v, tau = sampler.sample(st["theta_hat"])
+            store.add(th, u, v, tau)
+        ends.append(st)
+    log(f"DNS N={Nf}: {n_real} realizations x {n_snap} snapshots")
+    n_train = (n_real - n_test) * n_snap
+    store.count = n_train
+    in_scale, tgt_scale = store.scales()
+    net = C.ClosureNet(Nc, Nc, max_batch=32, device=dev, seed=0)
+    hist, train_sec = timed(lambda: C.train(net, store, 800, batch=32, lr=1e-3,
+                                            in_scale=in_scale, tgt_scale=tgt_scale, seed=1))
+    log(f"trained in {train_sec:.1f} s: loss {hist[-1][1]:.3e}")
+    plain = pg.gSQGSolver(gc, alpha, nu, 8, scheme="if")
+    closed = {m: C.ClosedSolver(gc, alpha, nu, 8, net=net, in_scale=in_scale,
+                                out_scale=1.0 / tgt_scale, scheme="if", every=m)
+              for m in every_list}
+    kmax = Nc // 3
+    bands = ((1, 20), (40, 69), (70, 85))
+    n_c = int(round(n_eval * dt / dt_c))
+
+    def shell(th_c):
+        return pg.compute_shell_spectrum(th_c, gc, alpha)[: kmax + 1]
+
+    def coarse_run(sol, thb0):
+        s0 = sol.initialize()
+        s0["theta_hat"] = thb0.clone()
+        sol.multistep(s0, 4, dt_c)  # plan set-up and graph capture outside the clock
+        return timed(lambda: sol.multistep(s0, n_c, dt_c))
+
+    per_real, secs = [], {"dns": [], "unclosed": []}
+    for m in every_list:
+        secs[f"closed_every_{m}"] = []
+    for r, st in enumerate(ends[n_real - n_test:]):
+        thb0 = C.spectral_filter(st["theta_hat"], gc)
+        dns.multistep(st, 4, dt)
+        st1, t_dns = timed(lambda: dns.multistep(st, n_eval, dt))
+        secs["dns"].append(t_dns)
+        E_dns = shell(C.spectral_filter(st1["theta_hat"], gc))
+        row = {}
+        for name, sol in [("unclosed", plain)] + [(f"closed_every_{m}", c)
+                                                   for m, c in closed.items()]:
+            s1, sec = coarse_run(sol, thb0)
+            secs[name].append(sec)
+            E = shell(s1["theta_hat"])
+            row[name] = {f"{lo}-{hi}": float(np.mean(np.log(E[lo:hi + 1] / E_dns[lo:hi + 1])))
+                         for lo, hi in bands}
+        per_real.append(row)
+        log(f"held-out realization {r}: " + "; ".join(
+            f"{k} " + ", ".join(f"{b} {v:+.3f}" for b, v in e.items()) for k, e in row.items()))
+    mean_s = {k: float(np.mean(v)) for k, v in secs.items()}
+    first = f"closed_every_{every_list[0]}"
+    out = {
+        "metric": "closure speedup at matched E(k) error",
+        "value": mean_s["dns"] / mean_s[first],
+        "unit": "x (filtered-DNS wall-clock / closed-run wall-clock, same simulated time)",
+        "n_gpus": 1, "steps": n_c, "warmup": 4,
+        "ms_per_step": mean_s[first] / n_c * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 solver, bf16/fp32 network",
+        "data": "synthetic (the solver's own decaying DNS)",
+        "config": {"workload": f"learned closure, decaying turbulence: DNS N={Nf} -> N={Nc}",
+                   "Nf": Nf, "Nc": Nc, "alpha": alpha, "nu_p": nu, "dt_dns": dt,
+                   "dt_coarse": dt_c, "realizations": n_real, "held_out": n_test,
+                   "snapshots_each": n_snap, "train_steps": 800, "t_eval": n_eval * dt,
+                   "closure_every": every_list},
+        "mean_log_E_ratio_vs_dns": per_real,
+        "wall_clock_s": mean_s,
+        "speedup": {k: mean_s["dns"] / v for k, v in mean_s.items() if k != "dns"},
+        "training": {"seconds": train_sec, "loss_last": hist[-1][1]},
         "cpu_baseline": None,
         "note": "cpu_baseline is reported for the default configuration only",
     }
