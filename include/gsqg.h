@@ -169,6 +169,23 @@ int gsqg_ifft2(gsqg_plan* plan, const void* spectrum_dev, double* field_dev, voi
 int gsqg_ring_forcing(gsqg_plan* plan, const void* theta_hat_dev, uint64_t key, void* out_dev,
                       void* stream);
 
+/* Number of entries in the plan's ring-mode list: the size of the OU state
+ * of gsqg_ring_forcing_ou is 2 * n complex128 values.  The reference keeps
+ * its state as a full (N, N) array (forcing/ring_forcing.py:61); only the
+ * ring modes of it are ever non-zero. */
+int gsqg_plan_ring_count(gsqg_plan* plan, int* n);
+
+/* RingForcing.__call__(theta_hat, key, dt) with tau_f > 0
+ * (forcing/ring_forcing.py:94-117, 144-145): the white ring noise of `key`
+ * goes through the Ornstein-Uhlenbeck filter
+ *     state = first ? white : decay * state + vfac * white,
+ * decay = exp(-dt / tau_f), vfac = sqrt(1 - decay^2), state updated in place
+ * (2 * gsqg_plan_ring_count values), and the filtered forcing is then made
+ * real at DC and the Nyquist corners and rescaled to <theta f> = epsilon as
+ * gsqg_ring_forcing does.  out_dev as there. */
+int gsqg_ring_forcing_ou(gsqg_plan* plan, const void* theta_hat_dev, uint64_t key, double decay,
+                         double vfac, void* ou_state_dev, int first, void* out_dev, void* stream);
+
 /* gsqg_advance without graphs, with a HIP event pair around every kernel:
  * class_ms[5] / class_count[5] receive the summed duration and the number
  * of launches of the theta column, theta row, scalar column, scalar row and

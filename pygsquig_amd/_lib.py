@@ -97,6 +97,9 @@ _SIGS = {
     "gsqg_fft2": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
     "gsqg_ifft2": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
     "gsqg_ring_forcing": (ctypes.c_int, [_VP, _VP, ctypes.c_uint64, _VP, _VP]),
+    "gsqg_plan_ring_count": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int)]),
+    "gsqg_ring_forcing_ou": (ctypes.c_int, [_VP, _VP, ctypes.c_uint64, ctypes.c_double,
+                                            ctypes.c_double, _VP, ctypes.c_int, _VP, _VP]),
     "gsqg_profile_advance": (ctypes.c_int, [_VP, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_uint64, ctypes.c_int64,
                                             ctypes.POINTER(ctypes.c_double),

@@ -309,6 +309,10 @@ class ClosedSolver(gSQGSolver):
         self.out_scale = float(out_scale)
         self.every = int(every)
 
+    def _step_generic(self, state, dt, forcing, damping, kwargs):
+        raise NotImplementedError("the closed solver runs the fused kernels only: pass a "
+                                  "RingForcing with tau_f = 0 and a CombinedDamping")
+
     def _plan(self, forcing, damping, combine=None, kappas=None) -> Plan:
         plan = super()._plan(forcing, damping, combine, kappas)
         if getattr(plan, "closure", None) is not self.net:

@@ -49,6 +49,7 @@ hipError_t launch_set_ctrl(Ctrl* c, int64_t step, uint64_t key, uint64_t seed, i
                            hipStream_t s);
 hipError_t launch_ring_field(const double2* X, const int2* modes, int n, int N, double amp,
                              double eps, uint64_t key, double2* out, double* rate_out,
+                             double2* ou, double decay, double vfac, int ou_first,
                              hipStream_t s);
 }  // namespace gsqg
 
@@ -894,7 +895,27 @@ extern "C" int gsqg_ring_forcing(gsqg_plan* p, const void* X, uint64_t key, void
   if (!p->ph.forcing) return fail(GSQG_ERR_ARG, "plan has no ring forcing");
   hipStream_t st = (hipStream_t)stream;
   HIPCHK(launch_ring_field((const double2*)X, p->ring, p->n_ring, p->N, p->ph.amp, p->ph.eps, key,
-                           (double2*)out, p->norms, st));
+                           (double2*)out, p->norms, nullptr, 0.0, 0.0, 0, st));
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_plan_ring_count(gsqg_plan* p, int* n) {
+  if (!p || !n) return fail(GSQG_ERR_ARG, "null argument");
+  *n = p->n_ring;
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_ring_forcing_ou(gsqg_plan* p, const void* X, uint64_t key, double decay,
+                                    double vfac, void* ou_state, int first, void* out,
+                                    void* stream) {
+  if (!p || !X || !out || !ou_state) return fail(GSQG_ERR_ARG, "null argument");
+  if (!p->ph.forcing) return fail(GSQG_ERR_ARG, "plan has no ring forcing");
+  if (!(decay >= 0.0 && decay <= 1.0) || !(vfac >= 0.0 && vfac <= 1.0))
+    return fail(GSQG_ERR_ARG, "decay and vfac must lie in [0, 1]");
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(launch_ring_field((const double2*)X, p->ring, p->n_ring, p->N, p->ph.amp, p->ph.eps, key,
+                           (double2*)out, p->norms, (double2*)ou_state, decay, vfac,
+                           first ? 1 : 0, st));
   return GSQG_OK;
 }
 

@@ -1453,11 +1453,66 @@ __global__ void a_rhs_kernel(Phys ph, const double* k1d, const uint8_t* flags, c
   }
 }
 
+// White ring noise w = amp * exp(i phi) at mode (i, jy) and at its mirror
+// (-i, -jy), before the DC / Nyquist realification (ring_forcing.py:138-141).
+__device__ __forceinline__ void ring_white(uint64_t key, int i, int jy, int N, double amp, cd& f1,
+                                           cd& f2) {
+  const int mi = (N - i) & (N - 1), mj = (N - jy) & (N - 1);
+  double s1, c1, s2, c2;
+  sincospi(2.0 * phase_u(key, (uint64_t)i * (uint64_t)N + (uint64_t)jy), &s1, &c1);
+  sincospi(2.0 * phase_u(key, (uint64_t)mi * (uint64_t)N + (uint64_t)mj), &s2, &c2);
+  f1 = cmk(amp * c1, amp * s1);
+  f2 = cmk(amp * c2, amp * s2);
+}
+
+// Ornstein-Uhlenbeck filter of one white value (ring_forcing.py:105-117):
+// the state becomes the white value on the first call, else
+// decay * state + vfac * white.  Returns the new state, which is also stored.
+__device__ __forceinline__ cd ou_update(cd* st, cd w, double decay, double vfac, int first) {
+  cd s;
+  if (first) {
+    s = w;
+  } else {
+    const cd o = *st;
+    s = cmk(__dadd_rn(__dmul_rn(decay, o.x), __dmul_rn(vfac, w.x)),
+            __dadd_rn(__dmul_rn(decay, o.y), __dmul_rn(vfac, w.y)));
+  }
+  *st = s;
+  return s;
+}
+
+// The forcing of ring entry e: Hermitian and anti-Hermitian parts of
+// F = amp * exp(i phi), or, with an OU state (two values per entry: the mode
+// and its mirror), of the filtered state after this call's update; DC and
+// the Nyquist corners made real afterwards (ring_forcing.py:138-150).
+__device__ __forceinline__ void ring_entry_parts(uint64_t key, int i, int jy, int N, double amp,
+                                                 cd* ou, int e, double decay, double vfac,
+                                                 int first, cd& PF, cd& QF) {
+  if (!ou) {
+    ring_parts(key, i, jy, N, amp, PF, QF);
+    return;
+  }
+  cd f1, f2;
+  ring_white(key, i, jy, N, amp, f1, f2);
+  f1 = ou_update(ou + 2 * (size_t)e, f1, decay, vfac, first);
+  f2 = ou_update(ou + 2 * (size_t)e + 1, f2, decay, vfac, first);
+  if ((i == 0 || i == N / 2) && (jy == 0 || jy == N / 2)) {
+    f1.y = 0.0;
+    f2 = f1;
+  }
+  PF = cscale(cadd(f1, cconj(f2)), 0.5);
+  QF = cscale(csub(f1, cconj(f2)), 0.5);
+}
+
 // RingForcing.__call__ on a full spectrum: rate by Parseval over the ring
-// list, then F = scale * amp * exp(i phi) at every ring mode
-// (ring_forcing.py:119-170).  One block; `out` must be zero-filled.
+// list, then F = scale * (raw forcing) at every ring mode
+// (ring_forcing.py:119-170).  With `ou` non-null the raw forcing is the OU
+// state (2 n values), updated once here (ring_forcing.py:144-145).  One
+// block; `out` must be zero-filled.  Each thread reads back only the OU
+// entries it updated itself.
 __global__ void ring_field_kernel(const cd* __restrict__ X, const int2* modes, int n, int N,
-                                  double amp, double eps, uint64_t key, cd* out, double* rate_out) {
+                                  double amp, double eps, uint64_t key, cd* out, double* rate_out,
+                                  cd* ou, double decay, double vfac, int ou_first) {
   __shared__ double red[256];
   double s = 0.0;
   for (int e = threadIdx.x; e < n; e += blockDim.x) {
@@ -1465,7 +1520,7 @@ __global__ void ring_field_kernel(const cd* __restrict__ X, const int2* modes, i
     const int mi = (N - i) & (N - 1), mj = (N - jy) & (N - 1);
     const cd H = cscale(cadd(X[(size_t)i * N + jy], cconj(X[(size_t)mi * N + mj])), 0.5);
     cd PF, QF;
-    ring_parts(key, i, jy, N, amp, PF, QF);
+    ring_entry_parts(key, i, jy, N, amp, ou, e, decay, vfac, ou_first, PF, QF);
     const double w = (jy == 0 || jy == N / 2) ? 1.0 : 2.0;
     s += w * (H.x * PF.x + H.y * PF.y);
   }
@@ -1483,7 +1538,17 @@ __global__ void ring_field_kernel(const cd* __restrict__ X, const int2* modes, i
     const int i = modes[e].x, jy = modes[e].y;
     const int mi = (N - i) & (N - 1), mj = (N - jy) & (N - 1);
     cd PF, QF;
-    ring_parts(key, i, jy, N, amp, PF, QF);
+    if (ou) {
+      cd f1 = ou[2 * (size_t)e], f2 = ou[2 * (size_t)e + 1];
+      if ((i == 0 || i == N / 2) && (jy == 0 || jy == N / 2)) {
+        f1.y = 0.0;
+        f2 = f1;
+      }
+      PF = cscale(cadd(f1, cconj(f2)), 0.5);
+      QF = cscale(csub(f1, cconj(f2)), 0.5);
+    } else {
+      ring_parts(key, i, jy, N, amp, PF, QF);
+    }
     // F(k) = PF + QF, F(-k) = conj(PF - QF)
     out[(size_t)i * N + jy] = cscale(cadd(PF, QF), sc);
     if (jy != 0 && jy != N / 2) out[(size_t)mi * N + mj] = cscale(cconj(csub(PF, QF)), sc);
@@ -1756,9 +1821,10 @@ hipError_t launch_a_rhs(const Phys& ph, const double* k1d, const uint8_t* flags,
   return hipGetLastError();
 }
 hipError_t launch_ring_field(const cd* X, const int2* modes, int n, int N, double amp, double eps,
-                             uint64_t key, cd* out, double* rate_out, hipStream_t s) {
+                             uint64_t key, cd* out, double* rate_out, cd* ou, double decay,
+                             double vfac, int ou_first, hipStream_t s) {
   hipLaunchKernelGGL(ring_field_kernel, dim3(1), dim3(256), 0, s, X, modes, n, N, amp, eps, key,
-                     out, rate_out);
+                     out, rate_out, ou, decay, vfac, ou_first);
   return hipGetLastError();
 }
 hipError_t launch_set_ctrl(Ctrl* c, int64_t step, uint64_t key, uint64_t seed, int key_mode,

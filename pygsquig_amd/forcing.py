@@ -23,6 +23,15 @@ class RingForcing:
     (:90); this implementation draws them from the SplitMix64 counter hash of
     ``pygsquig_amd.rng`` (same key -> same phases on host and device).  When no
     key is passed to ``step``, step ``n`` uses ``step_key(seed, n)``.
+
+    ``tau_f > 0`` makes the forcing an Ornstein-Uhlenbeck process (:94-117):
+    every call updates ``forcing_state`` in place, exactly as the reference's
+    calls do, so a solver step (four RK4 stages, each calling the forcing)
+    advances it four times, all with that step's key and dt.  The state is a
+    device tensor of the ring modes only (2 complex values per entry of the
+    ring list: the mode and its mirror); the reference's full (N, N) array is
+    zero everywhere else.  Such a forcing is not lowered into the fused
+    kernels: the solver calls it per stage.
     """
 
     def __init__(self, kf: float, dk: float = 1.0, epsilon: float = 0.1, tau_f: float = 0.0,
@@ -33,7 +42,10 @@ class RingForcing:
         self.tau_f = tau_f
         self.amplitude = float(np.sqrt(epsilon)) if amplitude is None else float(amplitude)
         self.seed = int(seed)
+        if tau_f < 0:
+            raise ValueError(f"tau_f must be non-negative, got {tau_f}")
         self.forcing_state = None
+        self._state_grid = None
 
     def signature(self):
         return ("ring", float(self.kf), float(self.dk), float(self.epsilon), float(self.amplitude))
@@ -47,14 +59,28 @@ class RingForcing:
         """F scaled so that <theta Re ifft2 F> = epsilon (:119-170)."""
         if grid is None:
             raise TypeError("RingForcing.__call__ needs grid=")
-        if self.tau_f != 0:
-            raise NotImplementedError("tau_f > 0 (stateful OU forcing) is not supported")
         from .plan import forcing_plan
 
         k = key_to_u64(key) if key is not None else self.seed
         th = torch.as_tensor(theta_hat).to(device=grid.device, dtype=torch.complex128).contiguous()
         out = torch.zeros_like(th)
-        forcing_plan(grid, self).ring_forcing(th, k, out)
+        plan = forcing_plan(grid, self)
+        if self.tau_f == 0:
+            plan.ring_forcing(th, k, out)
+            return out
+        if dt is None:
+            raise TypeError("RingForcing with tau_f > 0 needs dt=")
+        gkey = (grid.N, float(grid.L), str(grid.device))
+        first = self.forcing_state is None
+        if first:
+            self.forcing_state = torch.zeros((max(plan.ring_count(), 1), 2),
+                                             dtype=torch.complex128, device=grid.device)
+            self._state_grid = gkey
+        elif self._state_grid != gkey:
+            raise ValueError("forcing_state was built on another grid; reset it to None first")
+        decay = float(np.exp(-float(dt) / self.tau_f))
+        vfac = float(np.sqrt(1.0 - decay ** 2))
+        plan.ring_forcing_ou(th, k, decay, vfac, self.forcing_state, first, out)
         return out
 
 

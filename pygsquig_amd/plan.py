@@ -204,6 +204,20 @@ class Plan:
     def ifft2(self, X: torch.Tensor, f: torch.Tensor) -> None:
         _lib.check(self.lib.gsqg_ifft2(self.handle, _ptr(X), _ptr(f), _stream()), "gsqg_ifft2")
 
+    def ring_count(self) -> int:
+        n = ctypes.c_int(0)
+        _lib.check(self.lib.gsqg_plan_ring_count(self.handle, ctypes.byref(n)),
+                   "gsqg_plan_ring_count")
+        return int(n.value)
+
+    def ring_forcing_ou(self, th: torch.Tensor, key: int, decay: float, vfac: float,
+                        state: torch.Tensor, first: bool, out: torch.Tensor) -> None:
+        _lib.check(self.lib.gsqg_ring_forcing_ou(self.handle, _ptr(th),
+                                                 ctypes.c_uint64(key & 0xFFFFFFFFFFFFFFFF),
+                                                 float(decay), float(vfac), _ptr(state),
+                                                 1 if first else 0, _ptr(out), _stream()),
+                   "gsqg_ring_forcing_ou")
+
     def ring_forcing(self, th: torch.Tensor, key: int, out: torch.Tensor) -> None:
         _lib.check(self.lib.gsqg_ring_forcing(self.handle, _ptr(th),
                                               ctypes.c_uint64(key & 0xFFFFFFFFFFFFFFFF),
@@ -310,11 +324,16 @@ _FORCING_PLANS: dict = {}
 
 
 def forcing_plan(grid: Grid, forcing) -> Plan:
-    """Plan carrying a RingForcing's ring list (for RingForcing.__call__)."""
+    """Plan carrying a RingForcing's ring list (for RingForcing.__call__).
+    The plan holds the ring and the rescale parameters only; an OU forcing's
+    state stays with the forcing object, so one plan serves both kinds."""
+    from .forcing import RingForcing
+
     key = (grid.N, grid.L, str(grid.device), forcing.signature())
     p = _FORCING_PLANS.get(key)
     if p is None:
-        p = Plan(grid, 1.0, 0.0, 8, forcing=forcing)
+        ring = RingForcing(forcing.kf, forcing.dk, forcing.epsilon, amplitude=forcing.amplitude)
+        p = Plan(grid, 1.0, 0.0, 8, forcing=ring)
         _FORCING_PLANS[key] = p
     return p
 

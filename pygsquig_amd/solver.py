@@ -28,7 +28,7 @@ from . import _lib
 from .forcing import CombinedDamping, RingForcing
 from .grid import Grid, fft2, ifft2
 from .plan import Plan
-from .rng import key_to_u64
+from .rng import key_to_u64, step_key
 
 State = dict
 
@@ -149,12 +149,11 @@ class gSQGSolver:
         th = _to_device_c128(theta_hat, self.grid)
         fused_f, fused_d, generic = self._split_plugins(forcing, damping, kwargs)
         if generic:
-            out = torch.empty_like(th)
-            self._plan(None, None).rhs(th, out)
+            out = self._core_rhs_full(th)
             if forcing is not None:
-                out = out + self._call_plugin(forcing, th, kwargs, is_forcing=True)
+                out = out + self._call_plugin(forcing, th, kwargs)
             if damping is not None:
-                out = out + self._call_plugin(damping, th, kwargs, is_forcing=False)
+                out = out + self._call_plugin(damping, th, kwargs)
             return out
         key = kwargs.get("key")
         k = key_to_u64(key) if key is not None else (fused_f.seed if fused_f else 0)
@@ -162,18 +161,108 @@ class gSQGSolver:
         self._plan(fused_f, fused_d).rhs(th, out, k)
         return out
 
-    def _call_plugin(self, fn, th, kwargs, is_forcing):
+    def _call_plugin(self, fn, th, kwargs, dt=None, step=None):
+        """One per-stage call of a forcing or damping plug-in.  Our RingForcing
+        and CombinedDamping take the arguments the reference's call sites
+        bind (scripts/run.py:380-382): the key (``step_key(seed, step)`` when
+        none is given, as in the fused path), dt, the grid.  Any other
+        callable gets ``fn(theta_hat, **kwargs)`` (core/solver.py:174,178)."""
         if isinstance(fn, RingForcing):
-            return fn(th, key=kwargs.get("key", fn.seed), dt=kwargs.get("dt"), grid=self.grid)
+            key = kwargs.get("key")
+            if key is None:
+                key = step_key(fn.seed, step) if step is not None else fn.seed
+            return fn(th, key=key, dt=kwargs.get("dt", dt), grid=self.grid)
         if isinstance(fn, CombinedDamping):
             return fn(th, grid=self.grid)
         return fn(th, **kwargs)
 
+    def _advection_plan(self) -> Plan:
+        """Plan of the bare advection term fft2(-J) (no dissipation)."""
+        plan = self._plans.get("advection")
+        if plan is None:
+            plan = Plan(self.grid, self.alpha, 0.0, self.p)
+            self._plans["advection"] = plan
+        return plan
+
+    def _hv_operator(self) -> torch.Tensor:
+        """-nu_p |k|^p over the full plane, 0 at k = 0 (core/operators.py:202-205)."""
+        op = self._plans.get("hv_operator")
+        if op is None:
+            op = -self.nu_p * (self.grid.k2 ** (self.p / 2))
+            op[0, 0] = 0.0
+            self._plans["hv_operator"] = op
+        return op
+
+    def _core_rhs_full(self, th: torch.Tensor) -> torch.Tensor:
+        """advection_hat + hyperviscosity(theta_hat) of a full spectrum that
+        may carry an anti-Hermitian part (core/solver.py:28-57): the
+        advection sees Re ifft2(theta_hat) only, the hyperviscosity acts on
+        every mode."""
+        out = torch.empty_like(th)
+        self._advection_plan().rhs(th, out)
+        if self.nu_p != 0:
+            out = out + self._hv_operator() * th
+        return out
+
+    def _if_factors(self, dt: float, damping: Optional[CombinedDamping]):
+        """exp(L dt / 2), exp(L dt) of the integrating-factor scheme's linear
+        operator L: the solver's hyperviscosity plus the CombinedDamping's
+        terms, 0 at k = 0 except the large-scale drag."""
+        key = ("if", float(dt), damping.signature() if damping is not None else None)
+        e = self._plans.get(key)
+        if e is None:
+            g = self.grid
+            lin = self._hv_operator().clone()
+            if damping is not None:
+                ld = -damping.nu_p * (g.k2 ** (damping.p / 2))
+                ld[0, 0] = 0.0
+                if damping.mu != 0:
+                    # the mask as float64: a Python float times a bool tensor
+                    # would be promoted to float32
+                    drag = (torch.sqrt(g.k2) < damping.kf / 2).to(torch.float64)
+                    ld = ld - damping.mu * drag
+                lin = lin + ld
+            e = (torch.exp(lin * (0.5 * dt)), torch.exp(lin * dt))
+            self._plans[key] = e
+        return e
+
     def _step_generic(self, state, dt, forcing, damping, kwargs):
+        """One RK4 step with per-stage plug-in calls on full spectra.  The
+        explicit scheme is core/solver.py:182-214 with compute_rhs; the IF
+        scheme is the integrating-factor (Lawson) RK4 of the fused kernels,
+        with the solver's hyperviscosity and a CombinedDamping in its linear
+        operator and the forcing called on each stage state."""
         th = _to_device_c128(state["theta_hat"], self.grid)
+        step = int(_scalar_tensor(state.get("step", 0), torch.int64))
+        if self.scheme == "if":
+            if damping is not None and not isinstance(damping, CombinedDamping):
+                raise ValueError("the IF scheme needs a CombinedDamping object: a generic "
+                                 "damping callable has no linear operator to integrate")
+            e1, e2 = self._if_factors(dt, damping)
+
+            def nonlin(y):
+                out = torch.empty_like(y)
+                self._advection_plan().rhs(y, out)
+                if forcing is not None:
+                    out = out + self._call_plugin(forcing, y, kwargs, dt, step)
+                return out
+
+            k1 = nonlin(th)
+            k2 = nonlin(e1 * (th + (0.5 * dt) * k1))
+            k3 = nonlin(e1 * th + (0.5 * dt) * k2)
+            k4 = nonlin(e2 * th + dt * (e1 * k3))
+            acc = e2 * k1
+            acc = acc + 2.0 * (e1 * k2)
+            acc = acc + 2.0 * (e1 * k3)
+            return e2 * th + (dt / 6.0) * (acc + k4)
 
         def rhs(y):
-            return self.compute_rhs(y, forcing, damping, **kwargs)
+            out = self._core_rhs_full(y)
+            if forcing is not None:
+                out = out + self._call_plugin(forcing, y, kwargs, dt, step)
+            if damping is not None:
+                out = out + self._call_plugin(damping, y, kwargs, dt, step)
+            return out
 
         k1 = rhs(th)
         k2 = rhs(th + 0.5 * dt * k1)
