@@ -197,12 +197,13 @@ int gsqg_profile_advance(gsqg_plan* plan, double dt, int n_steps, int key_mode, 
  * FFTs are XLA's).  The column and row kernels are compiled in several
  * shapes, indexed 0 .. n_variants-1 with 0 the default: radix-8 schedules
  * (8 elements per lane) in the default 128-thread tile, a tile twice as
- * tall and one half as tall, and a radix-16 schedule (16 elements per lane,
- * half the lanes per transform, fewer exchange passes).  n_variants = 3 for
- * n <= 32 (radix-8 tiles), 4 for 64 <= n <= 512 (the three tiles and
- * radix 16), 3 for n = 1024 (default and taller radix-8 tile, radix 16), 2
- * above (radix 8 and radix 16); gsqg_plan_shape_radix reports the radix of
- * a shape index.  gsqg_autotune times
+ * tall and one half as tall, a radix-16 schedule (16 elements per lane,
+ * half the lanes per transform, fewer exchange passes) and a radix-4
+ * schedule (4 elements per lane, twice the lanes: two waves per SIMD where
+ * radix 8 has one).  n_variants = 3 for n <= 32 (radix-8 tiles), 5 for
+ * 64 <= n <= 512 (the three tiles, radix 16, radix 4), 4 for n = 1024
+ * (default and taller radix-8 tile, radix 16, radix 4), 2 above (radix 8
+ * and radix 16); gsqg_plan_shape_radix reports the radix of a shape index.  gsqg_autotune times
  * gsqg_advance over n_steps steps (graphs on, explicit key 0) for every
  * (column, row) shape pair, best of `repeats` runs each, keeps the fastest
  * pair in the plan and returns it in best[2]; ms_table (n_variants^2 host
@@ -213,8 +214,8 @@ int gsqg_profile_advance(gsqg_plan* plan, double dt, int n_steps, int key_mode, 
  * changing the pair synchronises the device and drops captured graphs.
  * Pairs of one radix give bit-identical states except with ring forcing,
  * whose <theta f> partial sums are then added in another order; radix-16
- * transforms round differently from radix-8 ones.  Both differences are at
- * the rounding level. */
+ * and radix-4 transforms round differently from radix-8 ones.  Both
+ * differences are at the rounding level. */
 int gsqg_autotune(gsqg_plan* plan, double dt, int n_steps, int repeats, int* best,
                   double* ms_table, void* stream);
 int gsqg_plan_set_shape(gsqg_plan* plan, int col_shape, int row_shape);

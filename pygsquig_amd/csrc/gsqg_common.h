@@ -241,6 +241,13 @@ struct TwGlobal {
     apply_w1248(twiddle<DIR>(tw, Ns * 16, jm), twiddle<DIR>(tw, Ns * 16, 2 * jm),
                 twiddle<DIR>(tw, Ns * 16, 4 * jm), twiddle<DIR>(tw, Ns * 16, 8 * jm), x);
   }
+  // Radix-4 pass: W^(r jm), r = 1..3, W = exp(DIR 2 pi i / (4 Ns)).
+  template <int DIR>
+  __device__ __forceinline__ void pass4(int Ns, int jm, cd& w1, cd& w2, cd& w3) const {
+    w1 = twiddle<DIR>(tw, Ns * 4, jm);
+    w2 = twiddle<DIR>(tw, Ns * 4, 2 * jm);
+    w3 = twiddle<DIR>(tw, Ns * 4, 3 * jm);
+  }
   template <int DIR>
   __device__ __forceinline__ cd fin2(int, int L, int j) const {
     return twiddle<DIR>(tw, L, j);
@@ -332,6 +339,15 @@ struct TwOct {
     const int s = LT / (16 * Ns);
     apply_w1248(dir<DIR>(o.fwd(jm * s)), dir<DIR>(o.fwd(2 * jm * s)), dir<DIR>(o.fwd(4 * jm * s)),
                 dir<DIR>(o.fwd(8 * jm * s)), x);
+  }
+  // Radix-4 pass: one table value w = W^jm, W = exp(DIR 2 pi i / (4 Ns)),
+  // and its powers w^2 = w w, w^3 = w w^2 (two products instead of two more
+  // octant lookups, whose index arithmetic costs more than the products).
+  template <int DIR>
+  __device__ __forceinline__ void pass4(int Ns, int jm, cd& w1, cd& w2, cd& w3) const {
+    w1 = dir<DIR>(o.fwd(jm * (LT / (4 * Ns))));
+    w2 = cmul(w1, w1);
+    w3 = cmul(w1, w2);
   }
   template <int DIR>
   __device__ __forceinline__ cd fin2(int, int, int j) const {
@@ -543,7 +559,80 @@ __device__ __forceinline__ void fft_core16(cd (&v)[G][16], cd* buf, int gs, int 
   }
 }
 
-// Dispatch on the elements per lane (8 or 16), deduced from v.
+// The same over L/4 lanes with 4 elements per lane (lane t holds elements
+// t + m*L/4, m = 0..3): radix-4 Stockham passes, then a radix-2 pass where
+// log2 L is odd (pairs q, q + 2).  Twice the lanes of the radix-8 layout for
+// the same transform: at grid sizes where one radix-8 wave per SIMD is all a
+// launch has, this gives each SIMD two waves to switch between.
+template <int L, int DIR, int G, class TW>
+__device__ __forceinline__ void fft_core4(cd (&v)[G][4], cd* buf, int gs, int t, const TW& tws) {
+  static_assert(L >= 4 && (L & (L - 1)) == 0, "power of two >= 4");
+  constexpr int Q = L / 4;
+  constexpr int LOG = __builtin_ctz(L);
+  constexpr int P4 = LOG / 2;
+  constexpr int REM = LOG % 2;
+  int Ns = 1;
+#pragma unroll
+  for (int p = 0; p < P4; ++p) {
+    if (p > 0) {
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) v[g][m] = buf[g * gs + lpad(t + m * Q)];
+      }
+    }
+    if (Ns > 1) {
+      cd w1, w2, w3;
+      tws.template pass4<DIR>(Ns, t % Ns, w1, w2, w3);
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        v[g][1] = cmul(v[g][1], w1);
+        v[g][2] = cmul(v[g][2], w2);
+        v[g][3] = cmul(v[g][3], w3);
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) dft4<DIR>(v[g][0], v[g][1], v[g][2], v[g][3]);
+    const bool last = (p == P4 - 1) && (REM == 0);
+    if (!last) {
+      __syncthreads();
+      const int base = (t / Ns) * Ns * 4 + (t % Ns);
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) buf[g * gs + lpad(base + r * Ns)] = v[g][r];
+      }
+      __syncthreads();
+    }
+    Ns *= 4;
+  }
+  if (REM == 1) {  // radix 2, Ns = L/2: pairs (q, q + 2)
+    if (P4 > 0) {
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) v[g][m] = buf[g * gs + lpad(t + m * Q)];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const cd w = tws.template fin2<DIR>(0, L, t + q * Q);
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        cd a = v[g][q], b = cmul(v[g][q + 2], w);
+        dft2<DIR>(a, b);
+        v[g][q] = a;
+        v[g][q + 2] = b;
+      }
+    }
+  }
+}
+
+// Dispatch on the elements per lane (4, 8 or 16), deduced from v.
+template <int L, int DIR, int G, class TW>
+__device__ __forceinline__ void fft_core(cd (&v)[G][4], cd* buf, int gs, int t, const TW& tws) {
+  fft_core4<L, DIR, G>(v, buf, gs, t, tws);
+}
 template <int L, int DIR, int G, class TW>
 __device__ __forceinline__ void fft_core(cd (&v)[G][8], cd* buf, int gs, int t, const TW& tws) {
   fft_core8<L, DIR, G>(v, buf, gs, t, tws);

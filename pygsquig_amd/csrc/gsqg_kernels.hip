@@ -31,14 +31,21 @@ namespace gsqg {
 // kernel) or x-rows (row kernel) of a block.  Variant 3 runs the radix-16
 // schedule (16 elements per lane, half the lanes per transform): fewer LDS
 // exchange passes and, at large N, blocks whose register budget holds a
-// whole row without spilling.  Which variants each N compiles is listed in
-// variant_id() below.
+// whole row without spilling.  Variant 4 runs the radix-4 schedule (4
+// elements per lane, twice the lanes of radix 8): where one radix-8 wave
+// per SIMD is all a launch has (N <= 1024), two waves per SIMD hide each
+// other's memory, LDS and FP64 latencies.  Which variants each N compiles
+// is listed in variant_id() below.
 template <int N, int V = 0>
 struct CKShape {
-  static constexpr int R = V == 3 ? 16 : 8;  // elements per lane
-  static constexpr int Q = N / R;            // lanes per row
+  static constexpr int R = V == 3 ? 16 : V == 4 ? 4 : 8;  // elements per lane
+  static constexpr int Q = N / R;                        // lanes per row
   static constexpr int R00 = (128 / Q) > 1 ? (128 / Q) : 1;
-  static constexpr int R0 = V == 1 ? 2 * R00 : (V == 2 && R00 > 1 ? R00 / 2 : R00);
+  // radix 4: one row per block down to one wave, then one wave per block
+  static constexpr int R0 = V == 1   ? 2 * R00
+                            : V == 2 ? (R00 > 1 ? R00 / 2 : R00)
+                            : V == 4 ? ((64 / Q) > 1 ? (64 / Q) : 1)
+                                     : R00;
   static constexpr int RPW = R0 < (N / 2) ? R0 : (N / 2);  // rows per block
   static constexpr int NT = RPW * Q;
   static constexpr int PADL = Padded<N>::value;
@@ -52,14 +59,16 @@ struct CKShape {
 template <int N, int V = 0>
 struct RKShape {
   static constexpr int M = N / 2;
-  static constexpr int R = V == 3 ? 16 : 8;  // elements per lane
-  static constexpr int Q = M / R;            // lanes per x-row
+  static constexpr int R = V == 3 ? 16 : V == 4 ? 4 : 8;  // elements per lane
+  static constexpr int Q = M / R;                        // lanes per x-row
   static constexpr int W00 = (128 / Q) > 1 ? (128 / Q) : 1;
   // Radix-16 tiles at 256 lanes per x-line take two x-lines, so that each
-  // row of a tile is 32 contiguous bytes of the ky-major fields.
+  // row of a tile is 32 contiguous bytes of the ky-major fields; radix-4
+  // tiles take two x-lines, or one wave's worth where x-lines are shorter.
   static constexpr int W0 = V == 1   ? 2 * W00
                             : V == 2 ? (W00 > 1 ? W00 / 2 : W00)
                             : V == 3 ? (Q >= 256 ? 2 : W00)
+                            : V == 4 ? ((64 / Q) > 2 ? (64 / Q) : 2)
                                      : W00;
   static constexpr int WX = W0 < N ? W0 : N;  // x-rows per block
   static constexpr int NT = WX * Q;
@@ -1633,16 +1642,17 @@ static hipError_t rk_by_job_v(int job, const RKArgs& a, int ny, hipStream_t s, c
 // Shape variants compiled for grid size N, in the autotuner's index order;
 // index 0 is the default.  Radix-8 tiles 0-2 where a block of 128 threads
 // holds several rows (at N = 1024 the default column block is one row, so
-// there is no smaller tile), and the radix-16 variant 3 from N = 64 up.
+// there is no smaller tile), and from N = 64 up the radix-16 variant 3 and,
+// up to N = 1024, the radix-4 variant 4.
 template <int N>
 constexpr int n_variants() {
-  return N <= 32 ? 3 : N <= 512 ? 4 : N == 1024 ? 3 : 2;
+  return N <= 32 ? 3 : N <= 512 ? 5 : N == 1024 ? 4 : 2;
 }
 template <int N>
 constexpr int variant_id(int k) {
   if (N >= 2048) return k == 0 ? 0 : 3;
-  if (N == 1024) return k == 2 ? 3 : k;
-  return k;  // 0, 1, 2 (and 3 from N = 64)
+  if (N == 1024) return k <= 1 ? k : k + 1;  // 0, 1, 3, 4
+  return k;  // 0, 1, 2 (and 3, 4 from N = 64)
 }
 
 template <int N, int K = 0>

@@ -1,9 +1,9 @@
 """Shape variants and the autotuner: every (column, row) shape pair advances
 a plan to the same state as the default shapes -- bit for bit between
-radix-8 tiles without forcing; to rounding where a radix-16 kernel takes
-part (its transforms round differently) or with ring forcing (the
-<theta f> partials are then summed in another order) -- that state matches
-the oracle, and gsqg_autotune restores the state it times with."""
+radix-8 tiles without forcing; to rounding where a radix-16 or radix-4
+kernel takes part (their transforms round differently) or with ring forcing
+(the <theta f> partials are then summed in another order) -- that state
+matches the oracle, and gsqg_autotune restores the state it times with."""
 
 import numpy as np
 import pytest
@@ -14,8 +14,9 @@ from oracle import reference_sqg as R
 
 pytestmark = pytest.mark.gpu
 TWO_PI = 2 * np.pi
-# shape variants per grid size (gsqg.h: radix-8 tiles, plus radix 16 from N = 64)
-N_VARIANTS = {16: 3, 32: 3, 64: 4, 128: 4, 256: 4, 512: 4, 1024: 3, 2048: 2, 4096: 2, 8192: 2}
+# shape variants per grid size (gsqg.h: radix-8 tiles, plus radix 16 from N = 64 and
+# radix 4 from N = 64 to 1024)
+N_VARIANTS = {16: 3, 32: 3, 64: 5, 128: 5, 256: 5, 512: 5, 1024: 4, 2048: 2, 4096: 2, 8192: 2}
 
 
 @pytest.fixture(scope="module")
@@ -61,7 +62,7 @@ def test_shapes_identical_and_match_oracle(pg, N):
     col, row, nv = plan.shape()
     assert (col, row) == (0, 0) and nv == N_VARIANTS[N]
     radix = [plan.shape_radix(k) for k in range(nv)]
-    assert radix[0] == 8 and radix[-1] == 16 and set(radix) == {8, 16}
+    assert radix[0] == 8 and set(radix) == {4, 8, 16}
     base = _run(plan, ths, dt, n)
     for c in range(nv):
         for r in range(nv):
@@ -154,7 +155,7 @@ def test_ensemble_autotune_and_validation(pg):
     ref = ens.multistep(st, 5, 1e-3)
     tuned = ens.autotune(1e-3, n_steps=4, repeats=1)
     assert set(tuned) == {64, 4096}
-    assert len(tuned[64][1]) == 16
+    assert len(tuned[64][1]) == 25
     assert len(tuned[4096][1]) == 4  # radix 8 and radix 16 above N = 1024
     got = ens.multistep(st, 5, 1e-3)
     for a, b in zip(got["theta_hat"], ref["theta_hat"]):
@@ -183,7 +184,7 @@ def test_solver_autotune_with_scalars(pg):
         st = s.initialize(theta0, scalar_init=init)
         if tune:
             best, times = s.autotune(dt, forcing=f, damping=d, n_steps=4, repeats=1)
-            assert s._plan(f, d).shape()[:2] == best and len(times) == 16
+            assert s._plan(f, d).shape()[:2] == best and len(times) == 25
         out.append(s.multistep(st, n, dt, forcing=f, damping=d))
     a, b = out
     rel = float(torch.linalg.norm(a["theta_hat"] - b["theta_hat"]) / torch.linalg.norm(b["theta_hat"]))

@@ -123,7 +123,7 @@ def forced1024_oracle():
     return sto["theta_hat"], sc
 
 
-@pytest.mark.parametrize("shapes", [(0, 0), (0, 1), (1, 1), (2, 2), (0, 2)])
+@pytest.mark.parametrize("shapes", [(0, 0), (0, 1), (1, 1), (2, 2), (0, 2), (3, 3), (3, 0)])
 def test_forced_1024_workload_vs_oracle(pg, forced1024_oracle, shapes):
     """The forced N = 1024 workload with two scalars (kf = 40, damping,
     Gaussian scalars, IF, dt = 5e-4), in each column/row tile shape pair the
