@@ -59,23 +59,20 @@ def test_single_shell_decay(pg, N, scheme):
     assert s.check_finite()
 
 
-def test_forced_scalar_step_vs_oracle_2048(pg):
-    """Two forced, damped steps with a passive scalar at N = 2048 (IF)."""
-    from pygsquig_amd.scalars import gSQGSolverWithScalars
+_F2048 = dict(N=2048, dt=5e-4, n=2)
 
-    N, dt, n = 2048, 5e-4, 2
+
+@pytest.fixture(scope="module")
+def forced2048_oracle():
+    """Oracle states after two forced, damped steps with a passive scalar at
+    N = 2048 (IF)."""
+    from oracle import rng as orng
+
+    N, dt, n = _F2048["N"], _F2048["dt"], _F2048["n"]
     nu_d = 1e-8 * (512 / N) ** 8
-    g = pg.make_grid(N, TWO_PI)
     go = R.make_grid(N, TWO_PI)
     theta0 = ics.kolmogorov_53(N, seed=9)
     s0 = ics.gaussian_blob(N, (np.pi, np.pi), TWO_PI / 16)
-    solver = gSQGSolverWithScalars(g, 1.0, 0.0, 8, scheme="if", passive_scalars={"a": {"kappa": 0.0}})
-    f = pg.RingForcing(kf=100, dk=2, epsilon=0.1, seed=3)
-    d = pg.CombinedDamping(mu=0.1, kf=100, nu_p=nu_d, p=8)
-    st = solver.initialize(torch.from_numpy(theta0).cuda(), scalar_init={"a": torch.from_numpy(s0)})
-    st = solver.multistep(st, n, dt, forcing=f, damping=d)
-    from oracle import rng as orng
-
     so = R.Solver(go, 1.0, 0.0, 8, scheme="if")
     fo = R.RingForcing(kf=100, dk=2, epsilon=0.1)
     do = R.CombinedDamping(mu=0.1, kf=100, nu_p=nu_d, p=8)
@@ -85,10 +82,29 @@ def test_forced_scalar_step_vs_oracle_2048(pg):
         k = orng.step_key(3, i)
         sto = so.step(sto, dt, forcing=lambda y, k=k: fo(y, key=k, grid=go), damping=do)
         sc = so.step_scalars(sc, sto["theta_hat"], dt, {"a": 0.0})
+    return theta0, s0, sto["theta_hat"], sc["a"]
+
+
+@pytest.mark.parametrize("shapes", [(0, 0), (1, 1), (0, 1), (1, 0)])
+def test_forced_scalar_step_vs_oracle_2048(pg, forced2048_oracle, shapes):
+    """Two forced, damped steps with a passive scalar at N = 2048 (IF), in
+    each pair of the two tile shapes there: radix 8 (0) and radix 16 (1)."""
+    from pygsquig_amd.scalars import gSQGSolverWithScalars
+
+    N, dt, n = _F2048["N"], _F2048["dt"], _F2048["n"]
+    nu_d = 1e-8 * (512 / N) ** 8
+    theta0, s0, th_o, sc_o = forced2048_oracle
+    g = pg.make_grid(N, TWO_PI)
+    solver = gSQGSolverWithScalars(g, 1.0, 0.0, 8, scheme="if", passive_scalars={"a": {"kappa": 0.0}})
+    f = pg.RingForcing(kf=100, dk=2, epsilon=0.1, seed=3)
+    d = pg.CombinedDamping(mu=0.1, kf=100, nu_p=nu_d, p=8)
+    solver._plan(f, d).set_shape(*shapes)
+    st = solver.initialize(torch.from_numpy(theta0).cuda(), scalar_init={"a": torch.from_numpy(s0)})
+    st = solver.multistep(st, n, dt, forcing=f, damping=d)
     got = st["theta_hat"].cpu().numpy()
-    assert np.linalg.norm(got - sto["theta_hat"]) / np.linalg.norm(sto["theta_hat"]) < 1e-12
+    assert np.linalg.norm(got - th_o) / np.linalg.norm(th_o) < 1e-12
     gs = st["scalar_state"].scalars["a"].cpu().numpy()
-    assert np.linalg.norm(gs - sc["a"]) / np.linalg.norm(sc["a"]) < 1e-12
+    assert np.linalg.norm(gs - sc_o) / np.linalg.norm(sc_o) < 1e-12
 
 
 _F1024 = dict(N=1024, dt=5e-4, n=3, kf=40, seed=42)
