@@ -219,6 +219,29 @@ int gsqg_profile_advance(gsqg_plan* plan, double dt, int n_steps, int key_mode, 
 int gsqg_autotune(gsqg_plan* plan, double dt, int n_steps, int repeats, int* best,
                   double* ms_table, void* stream);
 int gsqg_plan_set_shape(gsqg_plan* plan, int col_shape, int row_shape);
+
+/* Batched uneven ensembles: the plans of several grid sizes (each plan one
+ * size, its members the y-slices of its launches, each with its own alpha)
+ * advanced with ONE column launch and ONE row launch per RK4 stage for all
+ * of them, the steps replayed as HIP graphs.  Counterpart of the
+ * reference's ensemble step, one jax.pmap'd rk4_step of _compute_core_rhs
+ * per member (core/multi_gpu.py:118-137), which runs members of one grid
+ * size only.
+ *
+ * Plans must be unforced, without scalars or closure, with 64 <= n <= 1024
+ * (at most 6 plans).  The batch runs every plan with its default radix-8
+ * tile shapes whatever shapes the plan itself is set to, so each member's
+ * state is that plan's own gsqg_advance with shape pair (0, 0) and key mode
+ * GSQG_KEY_EXPLICIT, to rounding (the same arithmetic compiled into another
+ * kernel may contract other multiply-adds).  The batch uses the plans' state
+ * buffers: load members with gsqg_load_member first, store them after; do
+ * not use the plans elsewhere while a batch call runs on them. */
+typedef struct gsqg_batch gsqg_batch;
+int gsqg_batch_create(gsqg_plan* const* plans, int n_plans, gsqg_batch** out);
+int gsqg_batch_destroy(gsqg_batch* batch);
+/* n_steps RK4 steps of every member of every plan (use_graph: replay the
+ * middle steps from a captured 16-step graph). */
+int gsqg_batch_advance(gsqg_batch* batch, double dt, int n_steps, int use_graph, void* stream);
 int gsqg_plan_shape(const gsqg_plan* plan, int* col_shape, int* row_shape, int* n_variants);
 int gsqg_plan_shape_radix(const gsqg_plan* plan, int shape, int* radix);
 

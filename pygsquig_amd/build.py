@@ -53,6 +53,8 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
                           [f"-DGSQG_INSTANTIATE_N={n}"]))
         units.append((os.path.join(CSRC, "gsqg_kernels.hip"), os.path.join(tmp, "kcommon.o"),
                       ["-DGSQG_INSTANTIATE_COMMON=1"]))
+        units.append((os.path.join(CSRC, "gsqg_kernels.hip"), os.path.join(tmp, "kbatch.o"),
+                      ["-DGSQG_INSTANTIATE_BATCH=1"]))
         units.append((os.path.join(CSRC, "gsqg_api.hip"), os.path.join(tmp, "api.o"), []))
         units.append((os.path.join(CSRC, "closure_kernels.hip"), os.path.join(tmp, "closure.o"), []))
         jobs = jobs or min(len(units), max(1, (os.cpu_count() or 4)), 16)

@@ -51,6 +51,10 @@ hipError_t launch_ring_field(const double2* X, const int2* modes, int n, int N, 
                              double eps, uint64_t key, double2* out, double* rate_out,
                              double2* ou, double decay, double vfac, int ou_first,
                              hipStream_t s);
+hipError_t launch_ck_batch(int e, const CKBatch& m, hipStream_t s);
+hipError_t launch_rk_batch(const RKBatch& m, hipStream_t s);
+bool batch_supports(int N);
+int batch_blocks(int N, bool ck);
 }  // namespace gsqg
 
 using namespace gsqg;
@@ -1511,4 +1515,140 @@ extern "C" int gsqg_spectral_filter(const void* in, int nf, void* out, int nc, d
   HIPCHK(cl::launch_spectral_filter((const double2*)in, nf, (double2*)out, nc, scale, mask,
                                     (const double2*)add, (hipStream_t)stream));
   return GSQG_OK;
+}
+
+// ------------------------------------------------------------- batches
+// Uneven ensembles in one launch per RK4 stage (gsqg_batch_*).  The
+// schedule is gsqg_advance's for an unforced, scalar-free plan with the
+// explicit key mode -- prologue, then per step four (row pass, column pass)
+// pairs, the last column pass of the last step without a prologue -- with
+// each launch covering every plan.
+struct gsqg_batch {
+  std::vector<gsqg_plan*> plans;
+  hipStream_t cap = nullptr;
+  hipGraphExec_t gexec_k = nullptr;  // kGraphSteps captured steps
+  double g_dt = 0.0;
+};
+
+static int batch_ck(gsqg_batch* b, int e, int prologue, double dt, hipStream_t st) {
+  CKBatch m{};
+  m.n = (int)b->plans.size();
+  int blk = 0;
+  for (int g = 0; g < m.n; ++g) {
+    gsqg_plan* p = b->plans[g];
+    m.N[g] = p->N;
+    m.gx[g] = batch_blocks(p->N, true);
+    m.blk0[g] = blk;
+    m.a[g] = ck_theta(p, p->th, e, prologue, 0, 0, dt);
+    blk += m.gx[g] * p->n_mem;
+  }
+  m.blk0[m.n] = blk;
+  HIPCHK(launch_ck_batch(e, m, st));
+  return GSQG_OK;
+}
+
+static int batch_rk(gsqg_batch* b, hipStream_t st) {
+  RKBatch m{};
+  m.n = (int)b->plans.size();
+  int blk = 0;
+  for (int g = 0; g < m.n; ++g) {
+    gsqg_plan* p = b->plans[g];
+    m.N[g] = p->N;
+    m.gx[g] = batch_blocks(p->N, false);
+    m.blk0[g] = blk;
+    m.a[g] = rk_theta(p, 0, 0, false);
+    blk += m.gx[g] * p->n_mem;
+  }
+  m.blk0[m.n] = blk;
+  HIPCHK(launch_rk_batch(m, st));
+  return GSQG_OK;
+}
+
+// One step: four (row, column) stage pairs; the last column pass writes the
+// next step's prologue unless this is the final step.
+static int batch_step(gsqg_batch* b, double dt, bool next_prologue, hipStream_t st) {
+  int rc;
+  for (int e = 1; e <= 4; ++e) {
+    if ((rc = batch_rk(b, st))) return rc;
+    if ((rc = batch_ck(b, e, (e < 4 || next_prologue) ? 1 : 0, dt, st))) return rc;
+  }
+  return GSQG_OK;
+}
+
+static int batch_drop_graph(gsqg_batch* b) {
+  if (b->gexec_k) HIPCHK(hipGraphExecDestroy(b->gexec_k));
+  b->gexec_k = nullptr;
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_batch_create(gsqg_plan* const* plans, int n_plans, gsqg_batch** out) {
+  if (!plans || !out) return fail(GSQG_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (n_plans < 1 || n_plans > kBatchMaxPlans)
+    return fail(GSQG_ERR_ARG, "a batch holds 1 to " + std::to_string(kBatchMaxPlans) +
+                                  " plans, got " + std::to_string(n_plans));
+  for (int g = 0; g < n_plans; ++g) {
+    const gsqg_plan* p = plans[g];
+    if (!p) return fail(GSQG_ERR_ARG, "null plan in batch");
+    for (int h = 0; h < g; ++h)
+      if (plans[h] == p) return fail(GSQG_ERR_ARG, "a plan appears twice in the batch");
+    if (!batch_supports(p->N))
+      return fail(GSQG_ERR_UNSUPPORTED, "batched grids need 64 <= N <= 1024 (power of two), got " +
+                                            std::to_string(p->N));
+    if (p->ph.forcing || p->n_scalars > 0 || p->cl_net)
+      return fail(GSQG_ERR_UNSUPPORTED, "batched plans carry no forcing, scalars or closure");
+    if (p->device != plans[0]->device)
+      return fail(GSQG_ERR_ARG, "batched plans must live on one device");
+  }
+  gsqg_batch* b = new gsqg_batch();
+  b->plans.assign(plans, plans + n_plans);
+  hipError_t e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete b;
+    return fail(GSQG_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  *out = b;
+  return GSQG_OK;
+}
+
+extern "C" int gsqg_batch_destroy(gsqg_batch* b) {
+  if (!b) return GSQG_OK;
+  int rc = batch_drop_graph(b);
+  if (b->cap) (void)hipStreamDestroy(b->cap);
+  delete b;
+  return rc;
+}
+
+extern "C" int gsqg_batch_advance(gsqg_batch* b, double dt, int n_steps, int use_graph,
+                                  void* stream) {
+  if (!b) return fail(GSQG_ERR_ARG, "null batch");
+  if (n_steps < 0) return fail(GSQG_ERR_ARG, "n_steps must be >= 0");
+  if (n_steps == 0) return GSQG_OK;
+  hipStream_t st = (hipStream_t)stream;
+  int rc;
+  if ((rc = batch_ck(b, 0, 1, dt, st))) return rc;  // prologue of the first stage
+  // steps 0 .. n-2 end with the next step's prologue: whole 16-step graphs
+  // where they fit, the rest eagerly; the last step ends without one
+  int n_a = n_steps - 1;
+  if (use_graph && n_a >= kGraphSteps) {
+    if (b->gexec_k && b->g_dt != dt && (rc = batch_drop_graph(b))) return rc;
+    if (!b->gexec_k) {
+      hipGraph_t g;
+      HIPCHK(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+      for (int s = 0; s < kGraphSteps && rc == GSQG_OK; ++s) rc = batch_step(b, dt, true, b->cap);
+      hipError_t ce = hipStreamEndCapture(b->cap, &g);
+      if (rc) return rc;
+      if (ce != hipSuccess)
+        return fail(GSQG_ERR_HIP, std::string("capture: ") + hipGetErrorString(ce));
+      hipError_t ie = hipGraphInstantiate(&b->gexec_k, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess)
+        return fail(GSQG_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+      b->g_dt = dt;
+    }
+    for (; n_a >= kGraphSteps; n_a -= kGraphSteps) HIPCHK(hipGraphLaunch(b->gexec_k, st));
+  }
+  for (; n_a > 0; --n_a)
+    if ((rc = batch_step(b, dt, true, st))) return rc;
+  return batch_step(b, dt, false, st);
 }

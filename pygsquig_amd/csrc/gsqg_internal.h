@@ -122,4 +122,28 @@ struct RKArgs {
   int n_loop;            // RK_SCALAR, N >= 2048: scalars one block transforms in turn
 };
 
+// Several plans' column or row launches of one RK4 stage as one launch
+// (uneven ensembles: one plan per grid size).  Block b belongs to plan g with
+// blk0[g] <= b < blk0[g + 1]; within it, field slice by = (b - blk0[g]) / gx[g]
+// and block bx = (b - blk0[g]) % gx[g] of that plan's own launch.
+constexpr int kBatchMaxPlans = 6;
+struct CKBatch {
+  int n;
+  int N[kBatchMaxPlans];
+  int gx[kBatchMaxPlans];
+  int blk0[kBatchMaxPlans + 1];
+  CKArgs a[kBatchMaxPlans];
+};
+struct RKBatch {
+  int n;
+  int N[kBatchMaxPlans];
+  int gx[kBatchMaxPlans];
+  int blk0[kBatchMaxPlans + 1];
+  RKArgs a[kBatchMaxPlans];
+};
+
+// Host-side argument checks of the launchers (gsqg_kernels.hip).
+bool check_ck_args(int job, const CKArgs& a, int ny);
+bool check_rk_args(int job, const RKArgs& a, int ny);
+
 }  // namespace gsqg

@@ -440,376 +440,34 @@ __device__ __forceinline__ ModeOut row0_selfc_epilogue(int E, int r, const ModeI
   return o;
 }
 
+// LDS elements of a column-kernel block: the field regions and the twiddle
+// octant table (N <= 2048; one element otherwise).
+template <int N, int JOB, int V>
+struct CKLds {
+  using S = CKShape<N, V>;
+  static constexpr int NFL = (JOB == CK_FWD || JOB == CK_INV) ? 1 : NFields<JOB>::value;
+  static constexpr int FIELDS = (S::FP < NFL ? S::FP : NFL) * S::REGION;
+  static constexpr int TW = N <= 2048 ? OctTab<N>::SIZE : 1;
+};
+
+// One column-kernel block: ky rows bx * RPW .. of field slice by (gx blocks
+// per slice), with its LDS at lds (CKLds::FIELDS elements) and twl
+// (CKLds::TW).  ck_kernel runs it as block (blockIdx.x, blockIdx.y); the
+// multi-plan kernel of the batched path runs blocks of several plans in one
+// launch.
+template <int N, int JOB, int E, bool FORCED, int V>
+__device__ __forceinline__ void ck_body(const CKArgs& a, const int bx, const int by, const int gx,
+                                        cd* lds, cd* twl) {
+#include "ck_body.inc"
+}
+
 template <int N, int JOB, int E, bool FORCED, int V>
 __global__ void __launch_bounds__((CKShape<N, V>::NT)) ck_kernel(CKArgs a) {
-  using S = CKShape<N, V>;
-  constexpr int R = S::R;
-  constexpr int Q = S::Q;
-  constexpr int PADL = S::PADL;
-  constexpr int NT = S::NT;
-  constexpr int NFL = (JOB == CK_FWD || JOB == CK_INV) ? 1 : NFields<JOB>::value;
-  // Twiddles from an LDS octant table where it is small (N <= 2048).
-  constexpr bool TWL = N <= 2048;
-  __shared__ cd lds[(S::FP < NFL ? S::FP : NFL) * S::REGION];
-  __shared__ cd twl[TWL ? OctTab<N>::SIZE : 1];
-  double* red = reinterpret_cast<double*>(lds);  // reduction scratch, aliases region 0
-
-  const int tid = threadIdx.x;
-  const int rl = tid / Q;
-  const int t = tid - rl * Q;
-  const int j = blockIdx.x * S::RPW + rl;
-  const size_t fo = (size_t)blockIdx.y * (size_t)a.field_stride;
-  const size_t ro = fo + (size_t)j * N;
-  cd* buf = lds + rl * PADL;
-  const Phys& ph = a.ph;
-  const double* k1d = a.k1d;
-  const cd* __restrict__ tw = a.tw;
-  const bool row0 = (j == 0);
-  const bool blk0 = (blockIdx.x == 0);  // block-uniform: row 0 lives in block 0
-
-  cd v[R];
-
-  // ---------------------------------------------------- API transforms
-  if (JOB == CK_FWD || JOB == CK_INV) {
-    const cd* src = (JOB == CK_FWD) ? a.J : a.th;
-    cd* dst = (JOB == CK_FWD) ? a.th : a.F[0];
-#pragma unroll
-    for (int m = 0; m < R; ++m) v[m] = src[ro + t + m * Q];
-    if (JOB == CK_INV) {
-      if (blk0) {  // row 0: R layout -> packed Z
-        __syncthreads();
-        if (row0) {
-#pragma unroll
-          for (int m = 0; m < R; ++m) buf[lpad(t + m * Q)] = v[m];
-        }
-        __syncthreads();
-        if (row0) {
-#pragma unroll
-          for (int m = 0; m < R; ++m) v[m] = z_from_r(buf, t + m * Q, N);
-        }
-      }
-#pragma unroll
-      for (int m = 0; m < R; ++m) v[m] = cscale(v[m], ph.inv_n2);
-      fft_lanes<N, +1>(v, buf, t, tw);
-    } else {
-      fft_lanes<N, -1>(v, buf, t, tw);
-      if (blk0) {  // row 0: packed Z -> R layout
-        __syncthreads();
-        if (row0) {
-#pragma unroll
-          for (int m = 0; m < R; ++m) buf[lpad(t + m * Q)] = v[m];
-        }
-        __syncthreads();
-        if (row0) {
-#pragma unroll
-          for (int m = 0; m < R; ++m) v[m] = r_from_z(buf, t + m * Q, N);
-        }
-      }
-    }
-#pragma unroll
-    for (int m = 0; m < R; ++m) dst[ro + t + m * Q] = v[m];
-    return;
-  }
-
-  // ------------------------------------------------------------ loads
-  // Every global load of the kernel is issued here, before its first store:
-  // loads and stores share one in-order completion counter, so a load issued
-  // after a store could only be waited for together with that store.
-  const Ctrl* ctrl = a.ctrl;
-  const int64_t step = ctrl->step;
-  uint64_t key = 0;
-  double sc = 1.0;
-  if (FORCED) {
-    key = ctrl_key(ctrl, step);
-    if (E > 0) sc = ctrl->scale[E];
-  }
-  cd y[R];
-  // LEAN (1024-lane blocks with 128 registers per lane, or 16 elements per
-  // lane): the epilogue's per-mode state is loaded where it is used instead
-  // of up front, which avoids spilling at the cost of loads queued behind
-  // earlier stores.
-  constexpr bool LEAN = NT > 512 || R > 8;
-  constexpr int NPRE = LEAN ? 1 : R;
-  cd thv[NPRE], accv[NPRE], ysv[NPRE];
-  const bool need_ys = (ph.scheme == 0) && (E >= 3);
-  if (E > 0) {
-#pragma unroll
-    for (int m = 0; m < R; ++m) v[m] = a.J[ro + t + m * Q];
-    if (!LEAN) {
-#pragma unroll
-      for (int m = 0; m < NPRE; ++m) thv[m] = a.th[ro + t + m * Q];
-#pragma unroll
-      for (int m = 0; m < NPRE; ++m) accv[m] = (E >= 2) ? a.acc[ro + t + m * Q] : cmk(0.0, 0.0);
-#pragma unroll
-      for (int m = 0; m < NPRE; ++m) ysv[m] = need_ys ? a.ys[ro + t + m * Q] : cmk(0.0, 0.0);
-    }
-  } else {
-#pragma unroll
-    for (int m = 0; m < R; ++m) y[m] = a.th[ro + t + m * Q];
-  }
-  RowCtx<N, R, LEAN> ctx;
-  ctx.t = t;
-  ctx.dxn = ph.dxn;
-#pragma unroll
-  for (int m = 0; m < (R + 3) / 4; ++m) ctx.fbw[m] = 0u;
-#pragma unroll
-  for (int m = 0; m < R; ++m) {
-    if constexpr (!LEAN) ctx.kxs[m] = k1d[t + m * Q];
-    ctx.fbw[m >> 2] |= (uint32_t)a.flags[(size_t)j * N + t + m * Q] << (8 * (m & 3));
-  }
-  ctx.ky = k1d[j];
-  ctx.kyn = k1d[N / 2];
-  // Anti-Hermitian ring side state: this thread's first entry.
-  constexpr bool AUPD = (JOB == CK_THETA && E == 4 && FORCED);
-  const int ent0 = blockIdx.x + gridDim.x * tid;
-  ATab at0{};
-  cd A0 = cmk(0.0, 0.0);
-  if (AUPD && ent0 < a.n_a) {
-    const ATab* ap = a.a_tab + ent0;
-    at0.kx = ap->kx;
-    at0.ky = ap->ky;
-    at0.ii = ap->ii;
-    at0.jy = ap->jy;
-    at0.fl = ap->fl;
-    at0.dc = ap->dc;
-    A0 = a.a_val[ent0];
-  }
-  if constexpr (TWL) OctTab<N>::fill(twl, tw, tid, NT);
-  __syncthreads();
-  const TwOct<N, N> twS{{twl}};
-  const TwGlobal twG{tw};
-
-  // --------------------------------------------------------- epilogue
-  if (E > 0) {
-    if constexpr (TWL)
-      fft_core<N, -1, 1>(*reinterpret_cast<cd(*)[1][R]>(&v), buf, 0, t, twS);
-    else
-      fft_core<N, -1, 1>(*reinterpret_cast<cd(*)[1][R]>(&v), buf, 0, t, twG);
-
-    if (blk0) {  // row 0: forward-transformed Jacobian, packed Z -> R layout
-      __syncthreads();
-      if (row0) {
-#pragma unroll
-        for (int m = 0; m < R; ++m) buf[lpad(t + m * Q)] = v[m];
-      }
-      __syncthreads();
-      if (row0) {
-#pragma unroll
-        for (int m = 0; m < R; ++m) v[m] = r_from_z(buf, t + m * Q, N);
-      }
-    }
-
-    // With every table preloaded, no global load follows these stores, so
-    // they are issued as soon as each mode is done.
-#pragma unroll
-    for (int m = 0; m < R; ++m) {
-      ModeIn in;
-      in.J = v[m];
-      if (LEAN) {
-        const size_t o = ro + t + m * Q;
-        in.th = a.th[o];
-        in.acc = (E >= 2) ? a.acc[o] : cmk(0.0, 0.0);
-        in.ys = need_ys ? a.ys[o] : cmk(0.0, 0.0);
-      } else {
-        in.th = thv[m];
-        in.acc = accv[m];
-        in.ys = ysv[m];
-      }
-      const bool selfc = row0 && t == 0 && (m == 0 || m == R / 2);
-      ModeOut o;
-      if (!selfc) {
-        const MInfo mi = minfo<N, R>(ctx, m, t, j, row0);
-        cd sF = cmk(0.0, 0.0);
-        if (FORCED && (mi.fl & FLAG_RING)) {
-          cd PF, QF;
-          ring_parts(key, mi.ii, mi.jy, N, ph.amp, PF, QF);
-          sF = cscale(PF, sc);
-        }
-        o = mode_epilogue<JOB == CK_SCALAR>(E, in, k2_of(mi.kx, mi.ky), false, mi.fl,
-                                            ph, a.dt, FORCED, sF);
-      } else {
-        o = row0_selfc_epilogue<JOB, FORCED>(E, m * Q, in, ctx.kx(m), ctx.fb(m), ctx.ky, ctx.kyn,
-                                             ph, a.dt, key, sc);
-      }
-      const int i = t + m * Q;
-      if (E <= 3) a.acc[ro + i] = o.acc;
-      if ((E == 2 || E == 3) && ph.scheme == 0) a.ys[ro + i] = o.ys;
-      if (E == 4) a.th[ro + i] = o.th;
-      y[m] = o.y;
-    }
-    // Anti-Hermitian ring side state (final stage, theta only).
-    if (AUPD) {
-      const double s1 = ctrl->scale[1], s2 = ctrl->scale[2], s3 = ctrl->scale[3], s4 = ctrl->scale[4];
-      for (int ent = ent0; ent < a.n_a; ent += gridDim.x * NT) {
-        ATab at = at0;
-        cd A = A0;
-        if (ent != ent0) {  // entries beyond one per thread (never at the shipped sizes)
-          const ATab* ap = a.a_tab + ent;
-          at.kx = ap->kx;
-          at.ky = ap->ky;
-          at.ii = ap->ii;
-          at.jy = ap->jy;
-          at.fl = ap->fl;
-          at.dc = ap->dc;
-          A = a.a_val[ent];
-        }
-        const double k2 = k2_of(at.kx, at.ky);
-        const bool dc = at.dc != 0;
-        cd PF, QF;
-        ring_parts(key, at.ii, at.jy, N, ph.amp, PF, QF);
-        const cd zero = cmk(0.0, 0.0);
-        cd An;
-        if (ph.scheme == 0) {
-          const double hdt = 0.5 * a.dt;
-          cd k1 = explicit_rhs(zero, A, k2, dc, at.fl, ph, true, cscale(QF, s1));
-          cd acc = k1;
-          cd y2 = cadd(A, cscale(k1, hdt));
-          cd k2v = explicit_rhs(zero, y2, k2, dc, at.fl, ph, true, cscale(QF, s2));
-          acc = cadd(acc, cscale(k2v, 2.0));
-          cd y3 = cadd(A, cscale(k2v, hdt));
-          cd k3 = explicit_rhs(zero, y3, k2, dc, at.fl, ph, true, cscale(QF, s3));
-          acc = cadd(acc, cscale(k3, 2.0));
-          cd y4 = cadd(A, cscale(k3, a.dt));
-          cd k4v = explicit_rhs(zero, y4, k2, dc, at.fl, ph, true, cscale(QF, s4));
-          cd sm = cadd(acc, k4v);
-          cd d;
-          if (ph.combine == 0) {
-            cd tt = cscale(sm, a.dt);
-            d = cmk(tt.x / 6.0, tt.y / 6.0);
-          } else {
-            d = cscale(sm, a.dt / 6.0);
-          }
-          An = cadd(A, d);
-        } else {
-          const double L = lin_coeff(k2, dc, at.fl, ph);
-          const double e1 = exp(L * 0.5 * a.dt), e2 = e1 * e1;
-          cd acc = cscale(cscale(QF, s1), e2);
-          acc = cadd(acc, cscale(cscale(cscale(QF, s2), e1), 2.0));
-          acc = cadd(acc, cscale(cscale(cscale(QF, s3), e1), 2.0));
-          An = cadd(cscale(A, e2), cscale(cadd(acc, cscale(QF, s4)), a.dt / 6.0));
-        }
-        a.a_val[ent] = An;
-      }
-    }
-    if (E == 4 && JOB == CK_THETA) {
-      bool bad = false;
-#pragma unroll
-      for (int m = 0; m < R; ++m) bad |= !(isfinite(y[m].x) && isfinite(y[m].y));
-      if (bad) a.ctrl->nonfinite = 1.0;
-    }
-  }
-
-  // -------------------------------------------------------------- prologue
-  if (!a.prologue) return;
-  if (a.ystage) {  // the stage state itself, for the closure's features
-#pragma unroll
-    for (int m = 0; m < R; ++m) a.ystage[ro + t + m * Q] = y[m];
-  }
-
-  // <theta f> partials of the prologue state (ring_forcing.py:153-155 by
-  // Parseval over the ring modes; weight 2 for each +/-k pair).
-  if (FORCED && a.rate_slot > 0) {
-    const uint64_t rkey = ctrl_key(ctrl, a.rate_next_key ? step + 1 : step);
-    double accr = 0.0;
-#pragma unroll
-    for (int m = 0; m < R; ++m) {
-      const bool selfc = row0 && t == 0 && (m == 0 || m == R / 2);
-      if (!selfc) {
-        const MInfo mi = minfo<N, R>(ctx, m, t, j, row0);
-        if (mi.fl & FLAG_RING) {
-          cd PF, QF;
-          ring_parts(rkey, mi.ii, mi.jy, N, ph.amp, PF, QF);
-          accr += 2.0 * (y[m].x * PF.x + y[m].y * PF.y);
-        }
-      } else {
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int jy = h ? N / 2 : 0;
-          const unsigned fl = h ? (ctx.fb(m) >> 4) : (ctx.fb(m) & 15u);
-          if (fl & FLAG_RING) {
-            cd PF, QF;
-            ring_parts(rkey, m * Q, jy, N, ph.amp, PF, QF);
-            accr += (h ? y[m].y : y[m].x) * PF.x;
-          }
-        }
-      }
-    }
-    __syncthreads();  // region 0 (aliased by red) may still be read
-    red[tid] = accr;
-    __syncthreads();
-    for (int s = NT / 2; s > 0; s >>= 1) {
-      if (tid < s) red[tid] += red[tid + s];
-      __syncthreads();
-    }
-    if (tid == 0) a.part[(size_t)a.rate_slot * a.part_stride + blockIdx.x] = red[0];
-  }
-
-  // Multipliers (per mode; row 0 in R layout) and inverse x-FFTs, G fields
-  // at a time in lockstep.
-  constexpr int NF = NFields<JOB>::value;
-  constexpr int G = S::FP < NF ? S::FP : NF;
-  constexpr int RG = S::REGION;
-  // gSQG exponent of this field: per ensemble member (blockIdx.y) when given
-  const double alpha = a.member_alpha ? a.member_alpha[blockIdx.y] : ph.alpha;
-  const int akind = a.member_alpha ? alpha_kind_of(alpha) : ph.alpha_kind;
-#pragma unroll
-  for (int g0 = 0; g0 < NF; g0 += G) {
-    cd w[G][R];
-    const bool needK = (JOB == CK_VEL) || (JOB == CK_THETA && g0 + G > 2);
-    // Register-tight shapes scale y by |k|^-alpha in place before the first
-    // group that needs it (the groups without it come first), so that no
-    // per-mode K stays live across a transform.
-    constexpr bool KIN = LEAN;
-    if constexpr (KIN) {
-      if (needK && g0 == (JOB == CK_VEL ? 0 : 2)) {
-#pragma unroll
-        for (int m = 0; m < R; ++m) {
-          const MInfo mi = minfo<N, R>(ctx, m, t, j, row0);
-          y[m] = cscale(y[m], inv_frac(k2_of(mi.kx, mi.ky), alpha, akind));
-        }
-      }
-    }
-#pragma unroll
-    for (int m = 0; m < R; ++m) {
-      const bool selfc = row0 && t == 0 && (m == 0 || m == R / 2);  // their fields vanish
-      const MInfo mi = minfo<N, R>(ctx, m, t, j, row0);
-      const double K = KIN ? 1.0
-                           : needK ? inv_frac(k2_of(mi.kx, mi.ky), alpha, akind) : 0.0;
-#pragma unroll
-      for (int g = 0; g < G; ++g)
-        w[g][m] = selfc ? cmk(0.0, 0.0)
-                        : prologue_field<JOB>(g0 + g, y[m], mi.kx, mi.ky, K, mi.kxnyq, mi.kynyq,
-                                              ph);
-    }
-    if (blk0) {  // row 0: R layout -> packed Z
-      __syncthreads();
-      if (row0) {
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-#pragma unroll
-          for (int m = 0; m < R; ++m) buf[g * RG + lpad(t + m * Q)] = w[g][m];
-        }
-      }
-      __syncthreads();
-      if (row0) {
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-#pragma unroll
-          for (int m = 0; m < R; ++m) w[g][m] = z_from_r(buf + g * RG, t + m * Q, N);
-        }
-      }
-    }
-    if constexpr (TWL)
-      fft_core<N, +1, G>(w, buf, RG, t, twS);
-    else
-      fft_core<N, +1, G>(w, buf, RG, t, twG);
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-      cd* dst = a.F[g0 + g] + fo + (size_t)j * N;
-#pragma unroll
-      for (int m = 0; m < R; ++m) dst[t + m * Q] = w[g][m];
-    }
-  }
+  using LS = CKLds<N, JOB, V>;
+  __shared__ cd lds[LS::FIELDS];
+  __shared__ cd twl[LS::TW];
+  const unsigned bx = blockIdx.x, by = blockIdx.y, gx = gridDim.x;
+#include "ck_body.inc"
 }
 
 // --------------------------------------------------------------- row kernel
@@ -920,204 +578,29 @@ __device__ __forceinline__ void rk_store_r2c(cd (&v)[R], cd* dst, int x0, cd* ld
   }
 }
 
+// LDS elements of a row-kernel block (see CKLds).
+template <int N, int JOB, int V>
+struct RKLds {
+  using S = RKShape<N, V>;
+  static constexpr int NF = RKFields<JOB>::value;
+  static constexpr int FIELDS = (S::FP < NF ? S::FP : NF) * S::REGION;
+  static constexpr int TW = N <= 2048 ? OctTab<N>::SIZE : 1;
+};
+
+// One row-kernel block: x-tile bx of field slice by (see ck_body).
+template <int N, int JOB, int V>
+__device__ __forceinline__ void rk_body(const RKArgs& a, const int bx, const int by,
+                                        cd* lds, cd* twl) {
+#include "rk_body.inc"
+}
+
 template <int N, int JOB, int V>
 __global__ void __launch_bounds__((RKShape<N, V>::NT)) rk_kernel(RKArgs a) {
-  using S = RKShape<N, V>;
-  constexpr int M = S::M, Q = S::Q, WX = S::WX, NT = S::NT, RG = S::REGION, R = S::R;
-  constexpr int NF = RKFields<JOB>::value;
-  constexpr int G = S::FP < NF ? S::FP : NF;  // fields per lockstep group
-  constexpr bool THETA = (JOB == RK_THETA || JOB == RK_THETA_CL);
-  // Twiddles and pre/post-twist factors from one LDS octant table of
-  // W_N where it is small (N <= 2048); W_M = W_N^2.
-  constexpr bool TWL = N <= 2048;
-  __shared__ cd lds[G * RG];
-  __shared__ cd twl[TWL ? OctTab<N>::SIZE : 1];
-  double* red = reinterpret_cast<double*>(lds);  // block 0's reduction, before any tile
-  const int tid = threadIdx.x;
-  const int b = tid / Q;
-  const int t = tid - b * Q;
-  const int x0 = rk_tile<N, V>(blockIdx.x) * WX;
-  const int x = x0 + b;
-  const size_t fo = (size_t)blockIdx.y * (size_t)a.field_stride;
-  const cd* __restrict__ tw = a.tw;
-  cd* buf = lds + b * S::RSTR;
-
-  // Reduce the <theta f> partials of stage red_slot (block 0 only; every
-  // other block is independent of it).
-  if (a.red_slot > 0 && blockIdx.x == 0 && blockIdx.y == 0) {
-    double s = 0.0;
-    const double* p = a.part + (size_t)a.red_slot * a.part_stride;
-    for (int i = tid; i < a.nblk_ck; i += NT) s += p[i];
-    red[tid] = s;
-    __syncthreads();
-    for (int h = NT / 2; h > 0; h >>= 1) {
-      if (tid < h) red[tid] += red[tid + h];
-      __syncthreads();
-    }
-    if (tid == 0) {
-      const double inv = 1.0 / ((double)N * (double)N);
-      const double rate = (red[0] * inv) * inv;
-      a.ctrl->rate[a.red_slot] = rate;
-      a.ctrl->scale[a.red_slot] = fabs(rate) > 1e-10 ? a.eps / rate : 1.0;
-      if (a.inc_step) a.ctrl->step = a.ctrl->step + 1;
-    }
-  } else if (a.inc_step && blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) {
-    a.ctrl->step = a.ctrl->step + 1;
-  }
-
-  // Loads first (they share one in-order completion counter with stores):
-  // the first tile group, the frozen velocity, the LDS tables.
-  vd2 tmp[G * R];
-  cd v[R];
-  cd uu[R], vv[R];
-  if (JOB == RK_R2C) {
-    const cd* src = reinterpret_cast<const cd*>(a.phys_in + fo + (size_t)x * N);
-#pragma unroll
-    for (int m = 0; m < R; ++m) v[m] = src[t + m * Q];
-  } else {
-    const cd* src[G];
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-      const int q = THETA ? ((g & 1) ? 2 + (g >> 1) : (g >> 1)) : g;
-      src[g] = a.F[q] + fo;
-    }
-    rk_issue<N, V, G>(src, x0, tid, tmp);
-  }
-  // closure flux of this x-line as packed real pairs (y = 2k, 2k + 1)
-  typedef float vf2 __attribute__((vector_size(8)));
-  vf2 tz[JOB == RK_THETA_CL ? R : 1];
-  if (JOB == RK_THETA_CL) {
-    const vf2* tp = reinterpret_cast<const vf2*>(a.tau + (size_t)x * N);
-#pragma unroll
-    for (int m = 0; m < R; ++m) tz[m] = tp[t + m * Q];
-  }
-  if (JOB == RK_SCALAR) {
-    const cd* up = reinterpret_cast<const cd*>(a.uphys + (size_t)x * N);
-    const cd* vp = reinterpret_cast<const cd*>(a.vphys + (size_t)x * N);
-#pragma unroll
-    for (int m = 0; m < R; ++m) {
-      uu[m] = up[t + m * Q];
-      vv[m] = vp[t + m * Q];
-    }
-  }
-  if constexpr (TWL) OctTab<N>::fill(twl, tw, tid, NT);
-  const PreTw<N, TWL> pre{TWL ? twl : tw};
-  const TwOct<M, N> twS{{twl}};
-  const TwGlobal twG{tw};
-
-  if (JOB == RK_R2C) {
-    __syncthreads();
-    if constexpr (TWL)
-      fft_core<M, -1, 1>(*reinterpret_cast<cd(*)[1][R]>(&v), buf, 0, t, twS);
-    else
-      fft_core<M, -1, 1>(*reinterpret_cast<cd(*)[1][R]>(&v), buf, 0, t, twG);
-    rk_store_r2c<N, V>(v, a.Jout + fo, x0, lds, tid, b, t, pre);
-    return;
-  }
-
-  // Theta: fields in the order theta_x, u, theta_y, v (F[0], F[2], F[1],
-  // F[3]), J = u * theta_x + v * theta_y on packed real pairs
-  // z(k) = (r(2k), r(2k+1)); scalars: s_x, s_y against the frozen u, v.
-  //
-  // With n_loop > 1 (scalars, N >= 2048) one block transforms every scalar
-  // of its x-lines in turn against the velocity it loaded once; the loads of
-  // the next scalar's first tile go out ahead of this scalar's stores.
-  constexpr bool SLOOP = JOB == RK_SCALAR && N >= 2048;
-  const int nloop = (SLOOP && a.n_loop > 1) ? a.n_loop : 1;
-  for (int sc = 0; sc < nloop; ++sc) {
-  const size_t fos = fo + (size_t)sc * (size_t)a.field_stride;
-  cd jz[R], hold[R];
-#pragma unroll
-  for (int g0 = 0; g0 < NF; g0 += G) {
-    cd w[G][R];
-    rk_commit<N, V, G>(tmp, lds, tid, b, t, pre, w);
-    // The next group's loads go out ahead of this group's stores: for theta
-    // as soon as this group's tiles are in registers, so that they travel
-    // during this group's transforms; for the scalars (whose velocity
-    // occupies registers) after them.
-    if constexpr (THETA) {
-      if (g0 + G < NF) {
-        const cd* src[G];
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-          const int s = g0 + G + g;
-          src[g] = a.F[(s & 1) ? 2 + (s >> 1) : (s >> 1)] + fos;
-        }
-        rk_issue<N, V, G>(src, x0, tid, tmp);
-      }
-    }
-    if constexpr (TWL)
-      fft_core<M, +1, G>(w, buf, RG, t, twS);
-    else
-      fft_core<M, +1, G>(w, buf, RG, t, twG);
-    if constexpr (!THETA) {
-      if (g0 + G < NF) {
-        const cd* src[G];
-#pragma unroll
-        for (int g = 0; g < G; ++g) src[g] = a.F[g0 + G + g] + fos;
-        rk_issue<N, V, G>(src, x0, tid, tmp);
-      }
-    }
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-      const int s = g0 + g;
-      if (JOB == RK_C2R || JOB == RK_VEL) {
-        double* dstp = (JOB == RK_C2R) ? a.phys_out : (s == 0 ? a.uout : a.vout);
-        cd* dst = reinterpret_cast<cd*>(dstp + fos + (size_t)x * N);
-#pragma unroll
-        for (int m = 0; m < R; ++m) dst[t + m * Q] = w[g][m];
-      } else if (THETA) {
-        if ((s & 1) == 0) {
-#pragma unroll
-          for (int m = 0; m < R; ++m) hold[m] = w[g][m];
-        } else {
-          double* side = (s == 1) ? a.uout : a.vout;
-          if (side) {
-            cd* dst = reinterpret_cast<cd*>(side + (size_t)x * N);
-#pragma unroll
-            for (int m = 0; m < R; ++m) dst[t + m * Q] = w[g][m];
-          }
-          if (s == 1) {
-#pragma unroll
-            for (int m = 0; m < R; ++m)
-              jz[m] = cmk(w[g][m].x * hold[m].x, w[g][m].y * hold[m].y);
-          } else {
-#pragma unroll
-            for (int m = 0; m < R; ++m)
-              jz[m] = cmk(jz[m].x + w[g][m].x * hold[m].x, jz[m].y + w[g][m].y * hold[m].y);
-          }
-        }
-      } else if (s == 0) {
-#pragma unroll
-        for (int m = 0; m < R; ++m) jz[m] = cmk(uu[m].x * w[g][m].x, uu[m].y * w[g][m].y);
-      } else {
-#pragma unroll
-        for (int m = 0; m < R; ++m)
-          jz[m] = cmk(jz[m].x + vv[m].x * w[g][m].x, jz[m].y + vv[m].y * w[g][m].y);
-      }
-    }
-  }
-  if (JOB == RK_C2R || JOB == RK_VEL) return;
-  if (JOB == RK_THETA_CL) {
-#pragma unroll
-    for (int m = 0; m < R; ++m)
-      jz[m] = cmk(jz[m].x + (double)tz[m][0], jz[m].y + (double)tz[m][1]);
-  }
-  if (SLOOP && sc + 1 < nloop) {  // the next scalar's first tile group
-    const cd* src[G];
-#pragma unroll
-    for (int g = 0; g < G; ++g) src[g] = a.F[g] + fos + (size_t)a.field_stride;
-    rk_issue<N, V, G>(src, x0, tid, tmp);
-  }
-#pragma unroll
-  for (int m = 0; m < R; ++m) v[m] = jz[m];
-  __syncthreads();
-  if constexpr (TWL)
-    fft_core<M, -1, 1>(*reinterpret_cast<cd(*)[1][R]>(&v), buf, 0, t, twS);
-  else
-    fft_core<M, -1, 1>(*reinterpret_cast<cd(*)[1][R]>(&v), buf, 0, t, twG);
-  rk_store_r2c<N, V>(v, a.Jout + fos, x0, lds, tid, b, t, pre);
-  }
+  using LS = RKLds<N, JOB, V>;
+  __shared__ cd lds[LS::FIELDS];
+  __shared__ cd twl[LS::TW];
+  const unsigned bx = blockIdx.x, by = blockIdx.y;
+#include "rk_body.inc"
 }
 
 #if defined(GSQG_INSTANTIATE_COMMON)
@@ -1715,8 +1198,50 @@ GSQG_FOR_EACH_N(GSQG_EXTERN)
 #define GSQG_CASE(N_, CALL) \
   case N_: return CALL;
 
+// Host-side checks before a launch: every pointer the job and stage
+// dereference is set, and the launch has at least one field slice.  A
+// missing one is an error here rather than an invalid address on the GPU.
+bool check_ck_args(int job, const CKArgs& a, int ny) {
+  if (ny < 1 || !a.tw) return false;
+  if (job == CK_FWD) return a.J && a.th;
+  if (job == CK_INV) return a.th && a.F[0];
+  if (job != CK_THETA && job != CK_SCALAR && job != CK_VEL) return false;
+  if (!a.ctrl || !a.k1d || !a.flags || !a.th) return false;
+  const int e = a.e;
+  if (e < 0 || e > 4) return false;
+  if (e > 0 && (!a.J || !a.acc)) return false;
+  if (a.ph.scheme == 0 && e >= 2 && !a.ys) return false;
+  const bool forced = job == CK_THETA && a.ph.forcing;
+  if (forced && e == 4 && a.n_a > 0 && (!a.a_tab || !a.a_val)) return false;
+  if (a.prologue) {
+    const int nf = job == CK_THETA ? 4 : 2;
+    for (int q = 0; q < nf; ++q)
+      if (!a.F[q]) return false;
+    if (forced && a.rate_slot > 0 && !a.part) return false;
+  }
+  return true;
+}
+
+bool check_rk_args(int job, const RKArgs& a, int ny) {
+  if (ny < 1 || !a.tw) return false;
+  switch (job) {
+    case RK_R2C: return a.phys_in && a.Jout;
+    case RK_C2R: return a.F[0] && a.phys_out;
+    case RK_VEL: return a.F[0] && a.F[1] && a.uout && a.vout;
+    case RK_SCALAR: return a.F[0] && a.F[1] && a.uphys && a.vphys && a.Jout;
+    case RK_THETA:
+    case RK_THETA_CL:
+      if (!a.F[0] || !a.F[1] || !a.F[2] || !a.F[3] || !a.Jout) return false;
+      if ((a.red_slot > 0 || a.inc_step) && !a.ctrl) return false;
+      if (a.red_slot > 0 && !a.part) return false;
+      return job == RK_THETA || a.tau;
+  }
+  return false;
+}
+
 hipError_t launch_ck(int N, int job, const CKArgs& a, int ny, hipStream_t s, const LaunchEv* ev,
                      int var) {
+  if (!check_ck_args(job, a, ny)) return hipErrorInvalidValue;
   switch (N) {
 #define X(N_) GSQG_CASE(N_, ck_by_job<N_>(job, a, ny, s, ev, var))
     GSQG_FOR_EACH_N(X)
@@ -1726,6 +1251,7 @@ hipError_t launch_ck(int N, int job, const CKArgs& a, int ny, hipStream_t s, con
 }
 hipError_t launch_rk(int N, int job, const RKArgs& a, int ny, hipStream_t s, const LaunchEv* ev,
                      int var) {
+  if (!check_rk_args(job, a, ny)) return hipErrorInvalidValue;
   switch (N) {
 #define X(N_) GSQG_CASE(N_, rk_by_job<N_>(job, a, ny, s, ev, var))
     GSQG_FOR_EACH_N(X)
@@ -1845,5 +1371,161 @@ hipError_t launch_set_ctrl(Ctrl* c, int64_t step, uint64_t key, uint64_t seed, i
 
 
 #endif  // GSQG_INSTANTIATE_COMMON
+
+#if defined(GSQG_INSTANTIATE_BATCH)
+// ------------------------------------------------------ batched launches
+// The theta column and row passes of several unforced, scalar-free plans
+// (the groups of an uneven ensemble) in one launch per RK4 stage.  Every
+// plan runs the default radix-8 tile (shape variant 0), whose blocks are 128
+// threads for 64 <= N <= 1024, so one launch shape fits every group; a block
+// finds its plan from the block table and runs that plan's body.
+constexpr int kBatchNT = 128;
+template <int N>
+constexpr bool batch_n_ok() {
+  return CKShape<N, 0>::NT == kBatchNT && RKShape<N, 0>::NT == kBatchNT;
+}
+static_assert(batch_n_ok<64>() && batch_n_ok<128>() && batch_n_ok<256>() && batch_n_ok<512>() &&
+                  batch_n_ok<1024>(),
+              "batched tiles are 128 threads at every supported N");
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+template <template <int, int, int> class LS, int JOB>
+struct BatchLds {
+  static constexpr int FIELDS =
+      cmax(cmax(cmax(LS<64, JOB, 0>::FIELDS, LS<128, JOB, 0>::FIELDS),
+                cmax(LS<256, JOB, 0>::FIELDS, LS<512, JOB, 0>::FIELDS)),
+           LS<1024, JOB, 0>::FIELDS);
+  static constexpr int TW = cmax(cmax(cmax(LS<64, JOB, 0>::TW, LS<128, JOB, 0>::TW),
+                                      cmax(LS<256, JOB, 0>::TW, LS<512, JOB, 0>::TW)),
+                                 LS<1024, JOB, 0>::TW);
+};
+static_assert(sizeof(CKBatch) <= 4096 && sizeof(RKBatch) <= 4096, "kernel argument size");
+
+__device__ __forceinline__ int batch_plan(const int* blk0, int n, int b) {
+  int g = 0;
+  while (g + 1 < n && b >= blk0[g + 1]) ++g;
+  return g;
+}
+
+// NMAX: the largest grid size the kernel holds a body for.  The register
+// allocation is the largest body's, so launches without N = 1024 run the
+// kernel without that body (two waves per SIMD instead of one).
+template <int E, int NMAX>
+__global__ void __launch_bounds__(kBatchNT) ck_batch_kernel(CKBatch m) {
+  using LS = BatchLds<CKLds, CK_THETA>;
+  __shared__ cd lds[LS::FIELDS];
+  __shared__ cd twl[LS::TW];
+  const int b = blockIdx.x;
+  const int g = batch_plan(m.blk0, m.n, b);
+  const int gx = m.gx[g];
+  const int loc = b - m.blk0[g];
+  const int by = loc / gx, bx = loc - by * gx;
+  const CKArgs& a = m.a[g];
+  switch (m.N[g]) {
+    case 64: ck_body<64, CK_THETA, E, false, 0>(a, bx, by, gx, lds, twl); break;
+    case 128: ck_body<128, CK_THETA, E, false, 0>(a, bx, by, gx, lds, twl); break;
+    case 256: ck_body<256, CK_THETA, E, false, 0>(a, bx, by, gx, lds, twl); break;
+    case 512: ck_body<512, CK_THETA, E, false, 0>(a, bx, by, gx, lds, twl); break;
+    case 1024:
+      if constexpr (NMAX >= 1024) ck_body<1024, CK_THETA, E, false, 0>(a, bx, by, gx, lds, twl);
+      break;
+  }
+}
+
+template <int NMAX>
+__global__ void __launch_bounds__(kBatchNT) rk_batch_kernel(RKBatch m) {
+  using LS = BatchLds<RKLds, RK_THETA>;
+  __shared__ cd lds[LS::FIELDS];
+  __shared__ cd twl[LS::TW];
+  const int b = blockIdx.x;
+  const int g = batch_plan(m.blk0, m.n, b);
+  const int gx = m.gx[g];
+  const int loc = b - m.blk0[g];
+  const int by = loc / gx, bx = loc - by * gx;
+  const RKArgs& a = m.a[g];
+  switch (m.N[g]) {
+    case 64: rk_body<64, RK_THETA, 0>(a, bx, by, lds, twl); break;
+    case 128: rk_body<128, RK_THETA, 0>(a, bx, by, lds, twl); break;
+    case 256: rk_body<256, RK_THETA, 0>(a, bx, by, lds, twl); break;
+    case 512: rk_body<512, RK_THETA, 0>(a, bx, by, lds, twl); break;
+    case 1024:
+      if constexpr (NMAX >= 1024) rk_body<1024, RK_THETA, 0>(a, bx, by, lds, twl);
+      break;
+  }
+}
+
+bool batch_supports(int N) { return N == 64 || N == 128 || N == 256 || N == 512 || N == 1024; }
+
+// Blocks per field slice of the column (ck) or row pass at grid size N.
+int batch_blocks(int N, bool ck) {
+  switch (N) {
+    case 64: return ck ? 32 / CKShape<64, 0>::RPW : 64 / RKShape<64, 0>::WX;
+    case 128: return ck ? 64 / CKShape<128, 0>::RPW : 128 / RKShape<128, 0>::WX;
+    case 256: return ck ? 128 / CKShape<256, 0>::RPW : 256 / RKShape<256, 0>::WX;
+    case 512: return ck ? 256 / CKShape<512, 0>::RPW : 512 / RKShape<512, 0>::WX;
+    case 1024: return ck ? 512 / CKShape<1024, 0>::RPW : 1024 / RKShape<1024, 0>::WX;
+  }
+  return 0;
+}
+
+// Checks the block table against the plans before a launch: every block
+// belongs to exactly one plan's launch, and the count of field slices it
+// implies is what the plan's arguments are checked with.
+static bool batch_slices(const int* N, const int* gx, const int* blk0, int n, bool ck, int g,
+                         int* ny) {
+  if (n < 1 || n > kBatchMaxPlans || blk0[0] != 0) return false;
+  if (!batch_supports(N[g]) || gx[g] != batch_blocks(N[g], ck)) return false;
+  const int span = blk0[g + 1] - blk0[g];
+  if (span <= 0 || span % gx[g] != 0) return false;
+  *ny = span / gx[g];
+  return true;
+}
+
+template <class B>
+static bool batch_has_1024(const B& m) {
+  for (int g = 0; g < m.n; ++g)
+    if (m.N[g] == 1024) return true;
+  return false;
+}
+
+template <int NMAX>
+static hipError_t launch_ck_batch_n(int e, const CKBatch& m, hipStream_t s) {
+  const dim3 grid(m.blk0[m.n]);
+  switch (e) {
+    case 0: hipLaunchKernelGGL((ck_batch_kernel<0, NMAX>), grid, dim3(kBatchNT), 0, s, m); break;
+    case 1: hipLaunchKernelGGL((ck_batch_kernel<1, NMAX>), grid, dim3(kBatchNT), 0, s, m); break;
+    case 2: hipLaunchKernelGGL((ck_batch_kernel<2, NMAX>), grid, dim3(kBatchNT), 0, s, m); break;
+    case 3: hipLaunchKernelGGL((ck_batch_kernel<3, NMAX>), grid, dim3(kBatchNT), 0, s, m); break;
+    case 4: hipLaunchKernelGGL((ck_batch_kernel<4, NMAX>), grid, dim3(kBatchNT), 0, s, m); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_ck_batch(int e, const CKBatch& m, hipStream_t s) {
+  if (m.n < 1 || m.n > kBatchMaxPlans) return hipErrorInvalidValue;
+  for (int g = 0; g < m.n; ++g) {
+    int ny = 0;
+    if (!batch_slices(m.N, m.gx, m.blk0, m.n, true, g, &ny)) return hipErrorInvalidValue;
+    const CKArgs& a = m.a[g];
+    if (a.e != e || a.ph.forcing || !check_ck_args(CK_THETA, a, ny)) return hipErrorInvalidValue;
+  }
+  return batch_has_1024(m) ? launch_ck_batch_n<1024>(e, m, s) : launch_ck_batch_n<512>(e, m, s);
+}
+
+hipError_t launch_rk_batch(const RKBatch& m, hipStream_t s) {
+  if (m.n < 1 || m.n > kBatchMaxPlans) return hipErrorInvalidValue;
+  for (int g = 0; g < m.n; ++g) {
+    int ny = 0;
+    if (!batch_slices(m.N, m.gx, m.blk0, m.n, false, g, &ny)) return hipErrorInvalidValue;
+    const RKArgs& a = m.a[g];
+    if (a.red_slot != 0 || !check_rk_args(RK_THETA, a, ny)) return hipErrorInvalidValue;
+  }
+  if (batch_has_1024(m))
+    hipLaunchKernelGGL(rk_batch_kernel<1024>, dim3(m.blk0[m.n]), dim3(kBatchNT), 0, s, m);
+  else
+    hipLaunchKernelGGL(rk_batch_kernel<512>, dim3(m.blk0[m.n]), dim3(kBatchNT), 0, s, m);
+  return hipGetLastError();
+}
+#endif  // GSQG_INSTANTIATE_BATCH
 
 }  // namespace gsqg

@@ -5,14 +5,20 @@ The reference has no batched path; a user loops ``gSQGSolver.step`` over
 realizations (core/solver.py:182-214).  Here the members of one grid size
 share one plan: their fields are the y-slices of every launch (one column
 and one row launch per RK4 stage for the whole group, each member with its
-own alpha), and the steps replay as a HIP graph.  Groups of different sizes
-are issued on separate streams, so their kernels overlap on the device.
-Each member follows exactly the single-realization solver's step.
+own alpha), and the steps replay as a HIP graph.  With ``launch="streams"``
+groups of different sizes are issued on separate streams, so their kernels
+overlap on the device.  With ``launch="batch"`` every group's column pass
+of an RK4 stage is one launch and every group's row pass another
+(``gsqg_batch_advance``): blocks find their group from a block table.  Each
+member follows exactly the single-realization solver's step either way.
 """
 
 from __future__ import annotations
 
 from typing import Optional, Sequence
+
+import ctypes
+import weakref
 
 import numpy as np
 import torch
@@ -32,11 +38,14 @@ class EnsembleSolver:
     """
 
     def __init__(self, members: Sequence[tuple], L: float = 2 * np.pi, nu_p: float = 0.0,
-                 p: int = 8, scheme: str = "explicit", device: Optional[torch.device] = None):
+                 p: int = 8, scheme: str = "explicit", device: Optional[torch.device] = None,
+                 launch: str = "streams"):
         if not members:
             raise ValueError("an ensemble needs at least one member")
         if scheme not in _SCHEMES:
             raise ValueError(f"scheme must be one of {sorted(_SCHEMES)}, got {scheme!r}")
+        if launch not in ("streams", "batch"):
+            raise ValueError(f"launch must be 'streams' or 'batch', got {launch!r}")
         if p not in (2, 4, 8):
             raise ValueError(f"p must be 2, 4, or 8, got {p}")
         self.members = [(int(N), float(a)) for N, a in members]
@@ -55,6 +64,21 @@ class EnsembleSolver:
                               member_alphas=[self.members[i][1] for i in idx])
                       for N, idx in self.groups.items()}
         self.streams = {N: torch.cuda.Stream(self.device) for N in self.groups}
+        self.launch = launch
+        self.batch = None
+        if launch == "batch":
+            bad = [N for N in self.groups if N not in _BATCH_SIZES]
+            if bad:
+                raise ValueError(f"launch='batch' needs grid sizes in {_BATCH_SIZES}, got {bad}")
+            lib = _lib.load()
+            plans = list(self.plans.values())
+            arr = (ctypes.c_void_p * len(plans))(*(pl.handle.value for pl in plans))
+            handle = ctypes.c_void_p()
+            _lib.check(lib.gsqg_batch_create(arr, len(plans), ctypes.byref(handle)),
+                       "gsqg_batch_create")
+            self.batch = handle
+            # the plans outlive the batch: the finalizer holds them
+            self._batch_fin = weakref.finalize(self, _destroy_batch, lib, handle, plans)
 
     def initialize(self, thetas: Sequence) -> dict:
         """Physical initial fields, one (N_m, N_m) array per member."""
@@ -75,6 +99,8 @@ class EnsembleSolver:
         if len(ths) != len(self.members):
             raise ValueError("state does not match the ensemble's members")
         cur = torch.cuda.current_stream(self.device)
+        if self.batch is not None:
+            return self._multistep_batch(state, ths, n_steps, dt, cur)
         # inputs and outputs are allocated on the current stream; each group's
         # stream waits for it before use and the current stream waits for the
         # groups before anything here is released or read
@@ -97,6 +123,26 @@ class EnsembleSolver:
             t = t + dt
         return {"theta_hat": new, "time": t, "step": torch.as_tensor(state["step"]) + n_steps}
 
+    def _multistep_batch(self, state, ths, n_steps, dt, cur):
+        ins = [t.to(self.device, torch.complex128).contiguous() for t in ths]
+        new = [torch.empty((N, N), dtype=torch.complex128, device=self.device)
+               for N, _ in self.members]
+        with torch.cuda.device(self.device):
+            for N, idx in self.groups.items():
+                for k, i in enumerate(idx):
+                    self.plans[N].load_member(k, ins[i])
+            lib = _lib.load()
+            _lib.check(lib.gsqg_batch_advance(self.batch, float(dt), int(n_steps), 1,
+                                              ctypes.c_void_p(cur.cuda_stream)),
+                       "gsqg_batch_advance")
+            for N, idx in self.groups.items():
+                for k, i in enumerate(idx):
+                    self.plans[N].store_member(k, new[i])
+        t = torch.as_tensor(state["time"], dtype=torch.float64)
+        for _ in range(n_steps):
+            t = t + dt
+        return {"theta_hat": new, "time": t, "step": torch.as_tensor(state["step"]) + n_steps}
+
     def step(self, state: dict, dt: float) -> dict:
         return self.multistep(state, 1, dt)
 
@@ -104,8 +150,12 @@ class EnsembleSolver:
         """Pick each group's column/row tile shapes by timing every pair on
         the group's plan (``Plan.autotune``; each group alone, on its own
         stream).  Returns ``{N: ((col, row), {(col, row): ms})}``; results of
-        later steps are unchanged bit for bit, only their speed."""
+        later steps are unchanged bit for bit, only their speed.  The
+        one-launch batch runs fixed radix-8 tiles, so with ``launch="batch"``
+        there is nothing to tune and the result is empty."""
         out = {}
+        if self.batch is not None:
+            return out
         cur = torch.cuda.current_stream(self.device)
         for N, plan in self.plans.items():
             s = self.streams[N]
@@ -125,3 +175,14 @@ def _fft2(t: torch.Tensor, grid):
     from .grid import fft2
 
     return fft2(t, grid)
+
+
+# grid sizes the one-launch batch compiles (gsqg_batch_create)
+_BATCH_SIZES = (64, 128, 256, 512, 1024)
+
+
+def _destroy_batch(lib, handle, plans):
+    try:
+        lib.gsqg_batch_destroy(handle)
+    except Exception:  # pragma: no cover
+        pass
