@@ -94,6 +94,9 @@ def main():
     ap.add_argument("--scheme", default="if", choices=["if", "explicit"])
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-closure", action="store_true",
+                    help="default configuration: skip the learned-closure measurement that the "
+                         "JSON line carries as 'closure' (run after the timed region)")
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default tile shapes (forced configuration)")
@@ -266,6 +269,19 @@ def main():
         "shape_radix": [plan.shape_radix(k) for k in plan.shape()[:2]],
         "cpu_baseline": cpu,
     }
+    if not args.no_closure:
+        # the metric's second half, measured in the same run after the timed
+        # region: the learned closure's wall-clock gain over the DNS at the
+        # spectrum error it keeps (closure-decay workload, every 4 steps)
+        cd = bench_closure_decay(args, emit=False)
+        out["closure"] = {
+            "metric": cd["metric"], "value": cd["value"], "unit": cd["unit"],
+            "workload": cd["config"]["workload"],
+            "speedup": cd["speedup"], "wall_clock_s": cd["wall_clock_s"],
+            "mean_log_E_ratio_vs_dns": cd["mean_log_E_ratio_vs_dns"],
+            "training_s": cd["training"]["seconds"],
+            "full_config": "DNS N=4096 -> closed N=512: bench.py --config closure "
+                           "(minutes; not part of this run)"}
     print(json.dumps(out))
 
 
@@ -690,10 +706,11 @@ def bench_closure(args):
     print(json.dumps(out))
 
 
-def bench_closure_decay(args):
+def bench_closure_decay(args, emit=True):
     """Decaying DNS N=512 -> closed and unclosed N=256 runs on held-out
     realizations: mean log(E / E_DNS) per shell band and wall-clock per
-    simulated time unit."""
+    simulated time unit.  Prints the JSON line (emit) or returns it, its
+    progress lines then going to stderr."""
     import numpy as np
     import torch
 
@@ -711,7 +728,8 @@ def bench_closure_decay(args):
     t_start = time.perf_counter()
 
     def log(msg):
-        print(f"[closure-decay {time.perf_counter() - t_start:6.1f}s] {msg}", flush=True)
+        print(f"[closure-decay {time.perf_counter() - t_start:6.1f}s] {msg}", flush=True,
+              file=sys.stdout if emit else sys.stderr)
 
     def timed(fn):
         torch.cuda.synchronize()
@@ -800,6 +818,8 @@ def bench_closure_decay(args):
         "cpu_baseline": None,
         "note": "cpu_baseline is reported for the default configuration only",
     }
+    if not emit:
+        return out
     print(json.dumps(out))
 
 

@@ -207,7 +207,7 @@ extern "C" int gsqg_plan_create(const gsqg_plan_desc* d, gsqg_plan** out) {
   gsqg_plan* p = new gsqg_plan();
   p->N = N;
   HIPCHK(hipGetDevice(&p->device));
-  p->hsz = (size_t)(N / 2) * (size_t)N;
+  p->hsz = (size_t)(N / 2) * (size_t)mixed_pitch(N);
   Phys& ph = p->ph;
   ph.N = N;
   ph.inv_n2 = 1.0 / ((double)N * (double)N);
@@ -298,7 +298,7 @@ extern "C" int gsqg_plan_create(const gsqg_plan_desc* d, gsqg_plan** out) {
   }
   HIPCHK(hipMemcpy(p->tw, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(p->k1d, d->k1d, (size_t)N * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(p->flags, d->mode_flags, h, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p->flags, d->mode_flags, (size_t)(N / 2) * (size_t)N, hipMemcpyHostToDevice));
   if (p->n_ring > 0) {
     HIPCHK(hipMemcpy(p->ring, d->ring_modes, (size_t)p->n_ring * sizeof(int2),
                      hipMemcpyHostToDevice));

@@ -494,7 +494,7 @@ __device__ __forceinline__ void rk_issue(const cd* const (&src)[G], int x0, int 
   for (int e = 0; e < G * R; ++e) {
     const int l = tid + (e % R) * NT;
     const int jr = l / WX, bb = l - jr * WX;
-    tmp[e] = *reinterpret_cast<const vd2*>(src[e / R] + (size_t)jr * N + x0 + bb);
+    tmp[e] = *reinterpret_cast<const vd2*>(src[e / R] + (size_t)jr * mixed_pitch(N) + x0 + bb);
   }
 }
 
@@ -574,7 +574,7 @@ __device__ __forceinline__ void rk_store_r2c(cd (&v)[R], cd* dst, int x0, cd* ld
       const cd O = cscale(cmul_mi(csub(Z, Zp)), 0.5);
       X = cadd(E, cmul(pre(jr), O));
     }
-    dst[(size_t)jr * N + x0 + bb] = X;
+    dst[(size_t)jr * mixed_pitch(N) + x0 + bb] = X;
   }
 }
 
