@@ -154,6 +154,15 @@ int gsqg_advance_scalars(gsqg_plan* plan, const double* u_dev, const double* v_d
 int gsqg_rhs(gsqg_plan* plan, const void* theta_hat_dev, void* rhs_dev, uint64_t key,
              void* stream);
 
+/* rhs_dev = compute_passive_scalar_rhs(scalar_hat, u, v, grid, kappa) with no
+ * source (scalars/passive_scalar.py:63-103): fft2(-jacobian(ifft2(s), u, v))
+ * - kappa k^2 s_hat, with the kappa of the plan's scalar `idx` and the caller's
+ * physical (u, v).  Overwrites that scalar's device state.  The per-stage
+ * right-hand side of PassiveScalarEvolver.step (:201-232) when a source term
+ * is added on the host. */
+int gsqg_scalar_rhs(gsqg_plan* plan, int idx, const void* scalar_hat_dev, const double* u_dev,
+                    const double* v_dev, void* rhs_dev, void* stream);
+
 /* (u, v) physical from theta_hat (core/operators.py:158-181). */
 int gsqg_velocity(gsqg_plan* plan, const void* theta_hat_dev, double* u_dev, double* v_dev,
                   void* stream);

@@ -108,6 +108,7 @@ _SIGS = {
                                      ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
                                      _VP]),
     "gsqg_plan_set_shape": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int]),
+    "gsqg_scalar_rhs": (ctypes.c_int, [_VP, ctypes.c_int, _VP, _VP, _VP, _VP, _VP]),
     "gsqg_batch_create": (ctypes.c_int, [ctypes.POINTER(_VP), ctypes.c_int, ctypes.POINTER(_VP)]),
     "gsqg_batch_destroy": (ctypes.c_int, [_VP]),
     "gsqg_batch_advance": (ctypes.c_int, [_VP, ctypes.c_double, ctypes.c_int, ctypes.c_int, _VP]),

@@ -835,6 +835,55 @@ extern "C" int gsqg_rhs(gsqg_plan* p, const void* X, void* out, uint64_t key, vo
   return GSQG_OK;
 }
 
+extern "C" int gsqg_scalar_rhs(gsqg_plan* p, int idx, const void* X, const double* u,
+                               const double* v, void* out, void* stream) {
+  if (!p || !X || !u || !v || !out) return fail(GSQG_ERR_ARG, "null argument");
+  if (idx < 0 || idx >= p->n_scalars) return fail(GSQG_ERR_ARG, "scalar index out of range");
+  hipStream_t st = (hipStream_t)stream;
+  const size_t off = (size_t)idx * p->hsz;
+  int rc;
+  if ((rc = load_half(p, (const double2*)X, p->s_th + off, nullptr, false, st))) return rc;
+  // prologue (i kx s, i ky s), the row pass against (u, v), then the explicit
+  // stage-1 epilogue, which leaves k1 = rhs(s) in the accumulator
+  CKArgs a{};
+  a.J = p->s_J + off;
+  a.th = p->s_th + off;
+  a.acc = p->s_acc + off;
+  a.ys = p->s_ys + off;
+  a.F[0] = p->s_F0 + off;
+  a.F[1] = p->s_F1 + off;
+  a.flags = p->flags;
+  a.k1d = p->k1d;
+  a.tw = p->tw;
+  a.ctrl = p->ctrl;
+  a.part = p->part;
+  a.part_stride = p->part_stride;
+  a.ph = p->ph;
+  a.ph.forcing = 0;
+  a.ph.damping = 0;
+  a.ph.scheme = 0;
+  a.ph.kappa = p->kappa[idx];
+  a.e = 0;
+  a.prologue = 1;
+  HIPCHK(ck(p, CK_SCALAR, a, 1, st));
+  RKArgs r{};
+  r.F[0] = a.F[0];
+  r.F[1] = a.F[1];
+  r.uphys = u;
+  r.vphys = v;
+  r.Jout = a.J;
+  r.tw = p->tw;
+  r.ctrl = p->ctrl;
+  r.part = p->part;
+  r.n_loop = 1;
+  HIPCHK(rk(p, RK_SCALAR, r, 1, st));
+  a.e = 1;
+  a.prologue = 0;
+  HIPCHK(ck(p, CK_SCALAR, a, 1, st));
+  HIPCHK(launch_half_to_full(a.acc, (double2*)out, p->N, st));
+  return GSQG_OK;
+}
+
 extern "C" int gsqg_velocity(gsqg_plan* p, const void* X, double* u, double* v, void* stream) {
   if (!p || !X || !u || !v) return fail(GSQG_ERR_ARG, "null argument");
   hipStream_t st = (hipStream_t)stream;

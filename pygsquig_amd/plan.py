@@ -189,6 +189,13 @@ class Plan:
                                                  _stream()), "gsqg_advance_scalars")
 
     # ----------------------------------------------------------- operators
+    def scalar_rhs(self, idx: int, sh: torch.Tensor, u: torch.Tensor, v: torch.Tensor,
+                   out: torch.Tensor) -> None:
+        """Explicit right-hand side of scalar idx (no source) into out; the
+        scalar's device state is overwritten."""
+        _lib.check(self.lib.gsqg_scalar_rhs(self.handle, int(idx), _ptr(sh), _ptr(u), _ptr(v),
+                                            _ptr(out), _stream()), "gsqg_scalar_rhs")
+
     def rhs(self, th: torch.Tensor, out: torch.Tensor, key: int = 0) -> None:
         _lib.check(self.lib.gsqg_rhs(self.handle, _ptr(th), _ptr(out),
                                      ctypes.c_uint64(key & 0xFFFFFFFFFFFFFFFF), _stream()),
