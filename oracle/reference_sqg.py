@@ -27,7 +27,10 @@ named in brackets hold the checks):
 - RingForcing: <theta * Re ifft2(F)> = epsilon after rescaling, mask counts,
   |F| = amplitude on the ring, realified DC/Nyquist.
 - damping: mode selection and values per ``tests/test_damping.py``.
-- scalars: pure diffusion closed form, mass conservation.
+- scalars: pure diffusion closed form, mass conservation; with source terms,
+  the RK4 amplification polynomial of a linear source, a pointwise RK4 of a
+  quadratic one, a source held at the step's start time, and the coupled
+  sources' explicit update [test_oracle_sources.py].
 - diagnostics: single-mode closed forms.
 """
 
@@ -358,11 +361,51 @@ def scalar_advection_hat(scalar_hat, u, v, grid: Grid):
     return fft2(-jacobian(ifft2(scalar_hat), u, v, grid))
 
 
-def scalar_rhs(scalar_hat, u, v, grid: Grid, kappa: float):
-    """passive_scalar.py:63-103 without a source term."""
+def scalar_rhs(scalar_hat, u, v, grid: Grid, kappa: float, source_hat=None):
+    """passive_scalar.py:63-103: advection + diffusion (+ source_hat)."""
     adv_hat = scalar_advection_hat(scalar_hat, u, v, grid)
     diff = -kappa * grid.k2 * scalar_hat
-    return adv_hat + diff
+    rhs = adv_hat + diff
+    if source_hat is not None:
+        rhs = rhs + source_hat
+    return rhs
+
+
+def scalar_source_hat(scalar_hat, source_fn, grid: Grid, t: float):
+    """PassiveScalarEvolver.compute_source (passive_scalar.py:172-190):
+    fft2(S(ifft2(s_hat), grid, t))."""
+    return fft2(source_fn(ifft2(scalar_hat), grid, t))
+
+
+def scalar_step(scalar_hat, u, v, grid: Grid, kappa: float, dt: float, source_fn=None,
+                t: float = 0.0, scheme: str = "explicit"):
+    """PassiveScalarEvolver.step (passive_scalar.py:212-232): RK4 of
+    compute_rhs with frozen (u, v), the source at the step's start time t in
+    every stage.  scheme "if": Lawson RK4 with the diffusion exact (the
+    scheme the fused kernels run for an IF solver's scalars)."""
+    def src(y):
+        return None if source_fn is None else scalar_source_hat(y, source_fn, grid, t)
+
+    if scheme == "if":
+        def nonlin(y):
+            out = scalar_advection_hat(y, u, v, grid)
+            return out if source_fn is None else out + src(y)
+
+        return if_rk4_step(scalar_hat, nonlin, -kappa * grid.k2, dt)
+    return rk4_step(scalar_hat, lambda y: scalar_rhs(y, u, v, grid, kappa, src(y)), dt)
+
+
+def apply_coupled_sources(scalars: dict, coupled_sources, grid: Grid, t: float, dt: float):
+    """MultiSpeciesEvolver._apply_coupled_sources (passive_scalar.py:378-403):
+    s_hat += dt * fft2(S), every coupled source seeing the fields as they
+    were before any was added."""
+    out = dict(scalars)
+    phys = {name: ifft2(sh) for name, sh in scalars.items()}
+    for source_fn in coupled_sources:
+        for name, source in source_fn(phys, grid, t).items():
+            if name in out:
+                out[name] = out[name] + dt * fft2(source)
+    return out
 
 
 # ---------------------------------------------------------------------------

@@ -871,7 +871,7 @@ extern "C" int gsqg_scalar_rhs(gsqg_plan* p, int idx, const void* X, const doubl
   r.F[1] = a.F[1];
   r.uphys = u;
   r.vphys = v;
-  r.Jout = a.J;
+  r.Jout = p->s_J + off;
   r.tw = p->tw;
   r.ctrl = p->ctrl;
   r.part = p->part;

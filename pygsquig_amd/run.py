@@ -106,13 +106,14 @@ def build_simulation(cfg: RunConfig, checkpoint: Optional[str] = None,
     grid = make_grid(cfg.grid.N, cfg.grid.L, device)
     diss = cfg.solver.dissipation
     species = cfg.scalars.species if cfg.scalars is not None and cfg.scalars.enabled else []
-    for sp in species:
-        if sp.source is not None and sp.source.type != "none":
-            raise NotImplementedError(f"scalar {sp.name!r}: source terms are not supported")
     if species:
+        from .sources import source_from_config
+
         solver = gSQGSolverWithScalars(grid, cfg.solver.alpha, diss.nu_p, diss.p, scheme=ti.scheme,
-                                       passive_scalars={sp.name: {"kappa": sp.kappa}
-                                                        for sp in species})
+                                       passive_scalars={sp.name: {
+                                           "kappa": sp.kappa,
+                                           "source": source_from_config(sp.source)}
+                                           for sp in species})
     else:
         solver = gSQGSolver(grid, cfg.solver.alpha, diss.nu_p, diss.p, scheme=ti.scheme)
 
