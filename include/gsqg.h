@@ -209,8 +209,11 @@ int gsqg_profile_advance(gsqg_plan* plan, double dt, int n_steps, int key_mode, 
  * tall and one half as tall, a radix-16 schedule (16 elements per lane,
  * half the lanes per transform, fewer exchange passes) and a radix-4
  * schedule (4 elements per lane, twice the lanes: two waves per SIMD where
- * radix 8 has one).  n_variants = 3 for n <= 32 (radix-8 tiles), 5 for
- * 64 <= n <= 512 (the three tiles, radix 16, radix 4), 4 for n = 1024
+ * radix 8 has one), and for 64 <= n <= 512 a radix-4 column tile with two
+ * field groups (two sets of lanes run a row's epilogue side by side, then
+ * transform different prologue fields; its row tile is the radix-4 one).
+ * n_variants = 3 for n <= 32 (radix-8 tiles), 6 for 64 <= n <= 512 (the
+ * three tiles, radix 16, radix 4, field groups), 4 for n = 1024
  * (default and taller radix-8 tile, radix 16, radix 4), 2 above (radix 8
  * and radix 16); gsqg_plan_shape_radix reports the radix of a shape index.  gsqg_autotune times
  * gsqg_advance over n_steps steps (graphs on, explicit key 0) for every

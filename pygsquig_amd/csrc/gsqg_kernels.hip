@@ -38,16 +38,21 @@ namespace gsqg {
 // is listed in variant_id() below.
 template <int N, int V = 0>
 struct CKShape {
-  static constexpr int R = V == 3 ? 16 : V == 4 ? 4 : 8;  // elements per lane
+  static constexpr int R = V == 3 ? 16 : (V == 4 || V == 5) ? 4 : 8;  // elements per lane
   static constexpr int Q = N / R;                        // lanes per row
   static constexpr int R00 = (128 / Q) > 1 ? (128 / Q) : 1;
   // radix 4: one row per block down to one wave, then one wave per block
   static constexpr int R0 = V == 1   ? 2 * R00
                             : V == 2 ? (R00 > 1 ? R00 / 2 : R00)
                             : V == 4 ? ((64 / Q) > 1 ? (64 / Q) : 1)
+                            : V == 5 ? 1
                                      : R00;
   static constexpr int RPW = R0 < (N / 2) ? R0 : (N / 2);  // rows per block
-  static constexpr int NT = RPW * Q;
+  // Field groups (variant 5): FG sets of Q lanes run one row's epilogue
+  // side by side and then transform different prologue fields, so a row's
+  // inverse transforms run in parallel instead of one group after another.
+  static constexpr int FG = V == 5 ? 2 : 1;
+  static constexpr int NT = RPW * Q * FG;
   static constexpr int PADL = Padded<N>::value;
   static constexpr int REGION = RPW * PADL;  // LDS elements per field
   // Prologue fields transformed together: two where they fit in 80 KB of
@@ -59,7 +64,8 @@ struct CKShape {
 template <int N, int V = 0>
 struct RKShape {
   static constexpr int M = N / 2;
-  static constexpr int R = V == 3 ? 16 : V == 4 ? 4 : 8;  // elements per lane
+  // variant 5 pairs the field-group column tile with the radix-4 row tile
+  static constexpr int R = V == 3 ? 16 : (V == 4 || V == 5) ? 4 : 8;  // elements per lane
   static constexpr int Q = M / R;                        // lanes per x-row
   static constexpr int W00 = (128 / Q) > 1 ? (128 / Q) : 1;
   // Radix-16 tiles at 256 lanes per x-line take two x-lines, so that each
@@ -68,7 +74,7 @@ struct RKShape {
   static constexpr int W0 = V == 1   ? 2 * W00
                             : V == 2 ? (W00 > 1 ? W00 / 2 : W00)
                             : V == 3 ? (Q >= 256 ? 2 : W00)
-                            : V == 4 ? ((64 / Q) > 2 ? (64 / Q) : 2)
+                            : (V == 4 || V == 5) ? ((64 / Q) > 2 ? (64 / Q) : 2)
                                      : W00;
   static constexpr int WX = W0 < N ? W0 : N;  // x-rows per block
   static constexpr int NT = WX * Q;
@@ -446,7 +452,10 @@ template <int N, int JOB, int V>
 struct CKLds {
   using S = CKShape<N, V>;
   static constexpr int NFL = (JOB == CK_FWD || JOB == CK_INV) ? 1 : NFields<JOB>::value;
-  static constexpr int FIELDS = (S::FP < NFL ? S::FP : NFL) * S::REGION;
+  // field groups: one region per prologue field (each API transform group
+  // its own region)
+  static constexpr int FIELDS = S::FG > 1 ? (NFL > S::FG ? NFL : S::FG) * S::PADL
+                                          : (S::FP < NFL ? S::FP : NFL) * S::REGION;
   static constexpr int TW = N <= 2048 ? OctTab<N>::SIZE : 1;
 };
 
@@ -1129,13 +1138,13 @@ static hipError_t rk_by_job_v(int job, const RKArgs& a, int ny, hipStream_t s, c
 // up to N = 1024, the radix-4 variant 4.
 template <int N>
 constexpr int n_variants() {
-  return N <= 32 ? 3 : N <= 512 ? 5 : N == 1024 ? 4 : 2;
+  return N <= 32 ? 3 : N <= 512 ? 6 : N == 1024 ? 4 : 2;
 }
 template <int N>
 constexpr int variant_id(int k) {
   if (N >= 2048) return k == 0 ? 0 : 3;
   if (N == 1024) return k <= 1 ? k : k + 1;  // 0, 1, 3, 4
-  return k;  // 0, 1, 2 (and 3, 4 from N = 64)
+  return k;  // 0, 1, 2 (and 3, 4, 5 from N = 64)
 }
 
 template <int N, int K = 0>

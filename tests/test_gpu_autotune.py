@@ -14,9 +14,9 @@ from oracle import reference_sqg as R
 
 pytestmark = pytest.mark.gpu
 TWO_PI = 2 * np.pi
-# shape variants per grid size (gsqg.h: radix-8 tiles, plus radix 16 from N = 64 and
-# radix 4 from N = 64 to 1024)
-N_VARIANTS = {16: 3, 32: 3, 64: 5, 128: 5, 256: 5, 512: 5, 1024: 4, 2048: 2, 4096: 2, 8192: 2}
+# shape variants per grid size (gsqg.h: radix-8 tiles, plus radix 16 from N = 64,
+# radix 4 from N = 64 to 1024, and the field-group column tile from N = 64 to 512)
+N_VARIANTS = {16: 3, 32: 3, 64: 6, 128: 6, 256: 6, 512: 6, 1024: 4, 2048: 2, 4096: 2, 8192: 2}
 
 
 @pytest.fixture(scope="module")
@@ -155,7 +155,7 @@ def test_ensemble_autotune_and_validation(pg):
     ref = ens.multistep(st, 5, 1e-3)
     tuned = ens.autotune(1e-3, n_steps=4, repeats=1)
     assert set(tuned) == {64, 4096}
-    assert len(tuned[64][1]) == 25
+    assert len(tuned[64][1]) == 36
     assert len(tuned[4096][1]) == 4  # radix 8 and radix 16 above N = 1024
     got = ens.multistep(st, 5, 1e-3)
     for a, b in zip(got["theta_hat"], ref["theta_hat"]):
@@ -184,7 +184,7 @@ def test_solver_autotune_with_scalars(pg):
         st = s.initialize(theta0, scalar_init=init)
         if tune:
             best, times = s.autotune(dt, forcing=f, damping=d, n_steps=4, repeats=1)
-            assert s._plan(f, d).shape()[:2] == best and len(times) == 25
+            assert s._plan(f, d).shape()[:2] == best and len(times) == 36
         out.append(s.multistep(st, n, dt, forcing=f, damping=d))
     a, b = out
     rel = float(torch.linalg.norm(a["theta_hat"] - b["theta_hat"]) / torch.linalg.norm(b["theta_hat"]))
