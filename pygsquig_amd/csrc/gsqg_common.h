@@ -61,6 +61,33 @@ __device__ __forceinline__ double phase_u(uint64_t key, uint64_t idx) {
   return (double)(phase_bits(key, idx) >> 11) * 0x1.0p-53;
 }
 
+// ------------------------------------------------------- block reduction
+// Sum of one double per thread over a block of NT threads, finished in
+// wave 0: the levels whose partner lies in another wave go through LDS
+// (red, NT doubles) with a barrier each; the last ones (partner offset < 64)
+// are shuffles down the wave, with no barrier.  Every level adds
+// red[i] + red[i + s] exactly as a tree reduction over LDS does, so the sum
+// is the same bit for bit.  All threads of the block must call it; the
+// result is valid in thread 0.  The caller has made red free for writing.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* red, int tid) {
+  if constexpr (NT > 64) {
+    red[tid] = v;
+    __syncthreads();
+#pragma unroll
+    for (int s = NT / 2; s >= 64; s >>= 1) {
+      if (tid < s) red[tid] += red[tid + s];
+      __syncthreads();
+    }
+    v = red[tid];  // wave 0's lanes
+  }
+  if (tid < 64) {
+#pragma unroll
+    for (int s = (NT < 64 ? NT : 64) / 2; s > 0; s >>= 1) v += __shfl_down(v, s);
+  }
+  return v;
+}
+
 // ---------------------------------------------------------------- FFT
 // LDS index with one pad element per 8 to break the stride-8 bank pattern of
 // the first Stockham pass.
