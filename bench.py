@@ -361,6 +361,26 @@ def bench_single(args):
     kernels = {k: {"avg_us": round(ms / cnt * 1e3, 2), "launches": cnt,
                    "GBps": round(lb[k] / (ms / 1e3 / cnt) / 1e9, 1) if k in lb else None}
                for k, (ms, cnt) in prof.items() if cnt}
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import cpu_baseline
+
+        nc = cpu_baseline.host_cores()
+        if args.config == "plain":
+            r = cpu_baseline.time_plain(N=N, n_steps=100, workers=nc)
+            sample = (f"{r['steps']} timed steps (after 1 warm-up) of the same N={N} plain "
+                      f"configuration, NumPy/SciPy reference-structured c2c path, IF scheme, "
+                      f"scipy.fft workers={nc}, CPU {r['cpu']}")
+        else:
+            r = cpu_baseline.time_large_rhs(N=N, workers=nc, n_scalars=n_s, alpha=alpha)
+            sample = (f"one timed evaluation each of the theta right-hand side with ring "
+                      f"forcing and damping ({r['t_theta_rhs']:.1f} s), a scalar right-hand side "
+                      f"({r['t_scalar_rhs']:.1f} s) and the velocity ({r['t_velocity']:.1f} s) at "
+                      f"N={N}, summed as 4 (theta + {n_s} scalars) + velocity per step; "
+                      f"NumPy/SciPy reference-structured c2c path, scipy.fft workers={nc}, "
+                      f"CPU {r['cpu']}")
+        cpu = {"value": N * N / r["sec_per_step"], "unit": "grid-point RK4 steps/s",
+               "cores": nc, "kind": "port", "sample": sample, "sec_per_step": r["sec_per_step"]}
     print(json.dumps({
         "metric": "N^2*RK4-steps/s + HBM GB/s fp64", "value": N * N * sps,
         "unit": "grid-point RK4 steps/s", "n_gpus": 1, "steps": steps, "warmup": warm,
@@ -375,8 +395,7 @@ def bench_single(args):
             "ms_per_pair": {f"{c},{r}": round(ms, 4) for (c, r), ms in tuned[1].items()}},
         "shape_radix": [plan.shape_radix(k) for k in plan.shape()[:2]],
         "kernels": kernels,
-        "cpu_baseline": None,
-        "note": "cpu_baseline is reported for the default configuration only"}))
+        "cpu_baseline": cpu}))
 
 
 def bench_batch(args):
@@ -433,6 +452,18 @@ def bench_batch(args):
                               for k, s in zip(members, states)], loop_steps)
     loop_value = pts * loop_steps / lsec
     B = sum(step_bytes(N, 0) for N, _ in members)
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import cpu_baseline
+
+        nc = cpu_baseline.host_cores()
+        r = cpu_baseline.time_batch(members, n_steps=2, workers=nc, nu=nu, dt=dt)
+        cpu = {"value": pts / r["sec_per_step"], "unit": "member-grid-point RK4 steps/s",
+               "cores": nc, "kind": "port", "sec_per_step": r["sec_per_step"],
+               "sample": f"{r['steps']} timed steps (after 1 warm-up) of the oracle solver "
+                         f"looped over the same {len(members)} members, NumPy/SciPy "
+                         f"reference-structured c2c path, IF scheme, scipy.fft workers={nc}, "
+                         f"CPU {r['cpu']}"}
     print(json.dumps({
         "metric": "member-grid-point RK4 steps/s (batched uneven ensemble)", "value": value,
         "unit": "member-grid-point RK4 steps/s", "n_gpus": 1, "steps": steps,
@@ -455,8 +486,7 @@ def bench_batch(args):
                           "frac": B * steps / sec / PEAK_HBM, "bytes_per_step": B},
         "loop_baseline": {"value": loop_value, "unit": "member-grid-point RK4 steps/s",
                           "steps": loop_steps, "speedup": value / loop_value},
-        "cpu_baseline": None,
-        "note": "cpu_baseline is reported for the default configuration only"}))
+        "cpu_baseline": cpu}))
 
 
 def bench_cnn(args):

@@ -93,3 +93,88 @@ def time_forced_scalars(N: int = 1024, n_steps: int = 3, workers: int | None = N
                 "n_scalars": n_scalars, "scheme": scheme, "cpu": cpu_model()}
     finally:
         R.set_fft_engine(None)
+
+
+def time_plain(N: int = 256, n_steps: int = 100, workers: int | None = None,
+               nu: float = 1e-4, dt: float = 1e-3, budget_s: float = 20.0) -> dict:
+    """Seconds per RK4 step of the plain unforced configuration (theta0 =
+    sin(4x) cos(4y), IF scheme as the GPU run), after one warm-up step."""
+    workers = workers or host_cores()
+    R.set_fft_engine(ScipyFFT(workers))
+    try:
+        g = R.make_grid(N, 2 * np.pi)
+        s = R.Solver(g, 1.0, nu, 8, scheme="if")
+        st = s.step(s.initialize(ics.sin4(N)), dt)
+        t0 = time.perf_counter()
+        done = 0
+        for _ in range(n_steps):
+            st = s.step(st, dt)
+            done += 1
+            if time.perf_counter() - t0 > budget_s:
+                break
+        return {"sec_per_step": (time.perf_counter() - t0) / done, "steps": done,
+                "workers": workers, "N": N, "cpu": cpu_model()}
+    finally:
+        R.set_fft_engine(None)
+
+
+def time_large_rhs(N: int = 8192, workers: int | None = None, n_scalars: int = 4,
+                   alpha: float = 0.5) -> dict:
+    """One RK4 step of the large configuration is 4 stages of (theta RHS with
+    ring forcing and damping) plus the scalar step (4 stages of n_scalars
+    scalar RHS) plus one velocity.  A whole step takes minutes on the host, so
+    this times one evaluation of each kind and reports their per-step sum:
+    4 (t_theta + n_scalars t_scalar) + t_velocity."""
+    workers = workers or host_cores()
+    R.set_fft_engine(ScipyFFT(workers))
+    try:
+        L = 2 * np.pi
+        g = R.make_grid(N, L)
+        s = R.Solver(g, alpha, 0.0, 8, scheme="if")
+        f = R.RingForcing(kf=100, dk=2, epsilon=0.1)
+        d = R.CombinedDamping(mu=0.1, kf=100, nu_p=1e-8 * (512 / N) ** 8, p=8)
+        th = s.initialize(ics.kolmogorov_53(N, seed=2, alpha=alpha))["theta_hat"]
+        sh = R.fft2(ics.gaussian_blob(N, ics.SCALAR_CENTRES[0], L / 16))
+        key = rng.step_key(42, 1)
+        t0 = time.perf_counter()
+        rhs = R.core_rhs(th, g, alpha, 0.0, 8) + f(th, key=key, grid=g) + d(th, g)
+        t_theta = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        u, v = s.compute_velocity(th)
+        t_vel = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        rs = R.scalar_rhs(sh, u, v, g, 0.0)
+        t_scalar = time.perf_counter() - t0
+        del rhs, rs
+        sec = 4 * (t_theta + n_scalars * t_scalar) + t_vel
+        return {"sec_per_step": sec, "t_theta_rhs": t_theta, "t_scalar_rhs": t_scalar,
+                "t_velocity": t_vel, "workers": workers, "N": N, "cpu": cpu_model()}
+    finally:
+        R.set_fft_engine(None)
+
+
+def time_batch(members, n_steps: int = 2, workers: int | None = None, nu: float = 1e-4,
+               dt: float = 1e-3, budget_s: float = 25.0) -> dict:
+    """Seconds per RK4 step of looping the oracle solver over the members
+    (N, alpha) of the uneven ensemble (IF, as the GPU run), after one
+    warm-up step of every member."""
+    workers = workers or host_cores()
+    R.set_fft_engine(ScipyFFT(workers))
+    try:
+        grids, solvers, states = {}, [], []
+        for m, (N, a) in enumerate(members):
+            g = grids.setdefault(N, R.make_grid(N, 2 * np.pi))
+            s = R.Solver(g, a, nu, 8, scheme="if")
+            solvers.append(s)
+            states.append(s.step(s.initialize(ics.kolmogorov_53(N, seed=100 + m, alpha=a)), dt))
+        t0 = time.perf_counter()
+        done = 0
+        for _ in range(n_steps):
+            states = [s.step(st, dt) for s, st in zip(solvers, states)]
+            done += 1
+            if time.perf_counter() - t0 > budget_s:
+                break
+        return {"sec_per_step": (time.perf_counter() - t0) / done, "steps": done,
+                "workers": workers, "members": len(members), "cpu": cpu_model()}
+    finally:
+        R.set_fft_engine(None)

@@ -210,14 +210,30 @@ __global__ void __launch_bounds__(256, 2) conv_mid(const uint16_t* __restrict__ 
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[h][g][r] = 0.0f;
   const int lr = l & 31, lh = l >> 5;
+  // One kernel row (3 taps, 1536 16-byte chunks) is in LDS at a time; the
+  // next row's chunks of this thread (6) are loaded into registers while
+  // the current row's MFMAs run, and written to LDS after them.
+  constexpr int WCH = 3 * C * 8 / 256;
+  static_assert(WCH * 256 == 3 * C * 8, "weight chunks per thread");
+  u32x4 wn[WCH];
+#pragma unroll
+  for (int k = 0; k < WCH; ++k) {
+    const int e = tid + 256 * k, row = e >> 3, h = e & 7;
+    *reinterpret_cast<u32x4*>(lw + swz(row, h)) =
+        *reinterpret_cast<const u32x4*>(Wt + (size_t)row * C + 8 * h);
+  }
+  __syncthreads();
   for (int ky = 0; ky < 3; ++ky) {
-    if (ky > 0) __syncthreads();  // every wave is done with the previous kernel row
-    for (int e = tid; e < 3 * C * 8; e += 256) {
-      const int row = e >> 3, h = e & 7;
-      *reinterpret_cast<u32x4*>(lw + swz(row, h)) =
-          *reinterpret_cast<const u32x4*>(Wt + (size_t)(3 * ky * C + row) * C + 8 * h);
+    if (ky < 2) {
+#pragma unroll
+      for (int k = 0; k < WCH; ++k) {
+        const int e = tid + 256 * k, row = e >> 3, h = e & 7;
+        wn[k] = *reinterpret_cast<const u32x4*>(Wt + (size_t)(3 * (ky + 1) * C + row) * C + 8 * h);
+      }
     }
-    __syncthreads();
+    // keep those loads ahead of the MFMAs (the scheduler would sink them
+    // towards their use, after the loop, and expose their latency)
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int kx = 0; kx < 3; ++kx) {
       const int wr0 = kx * C + lr, wr1 = wr0 + 32;
@@ -234,6 +250,15 @@ __global__ void __launch_bounds__(256, 2) conv_mid(const uint16_t* __restrict__ 
         acc[1][0] = mfma_32x32x16(A1, B0, acc[1][0]);
         acc[1][1] = mfma_32x32x16(A1, B1, acc[1][1]);
       }
+    }
+    if (ky < 2) {
+      __syncthreads();  // every wave is done with this kernel row
+#pragma unroll
+      for (int k = 0; k < WCH; ++k) {
+        const int e = tid + 256 * k, row = e >> 3, h = e & 7;
+        *reinterpret_cast<u32x4*>(lw + swz(row, h)) = wn[k];
+      }
+      __syncthreads();
     }
   }
   // epilogue straight from the accumulators
