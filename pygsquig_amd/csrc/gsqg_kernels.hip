@@ -587,13 +587,23 @@ __device__ __forceinline__ void rk_store_r2c(cd (&v)[R], cd* dst, int x0, cd* ld
   }
 }
 
-// LDS elements of a row-kernel block (see CKLds).
+// Row-kernel twiddles and pre/post-twist factors from an LDS octant table
+// of W_N (W_M = W_N^2): for every job up to N = 2048, and above that for the
+// time-step jobs where the table fits beside the field tiles in 160 KB.
+// There the table (18 KB at N = 8192) costs no occupancy, since registers
+// hold those blocks to one per CU, and without it each FFT pass's twiddle
+// reads are global loads, which complete in issue order and so wait for the
+// next field's tile loads issued ahead of the transform.
 template <int N, int JOB, int V>
 struct RKLds {
   using S = RKShape<N, V>;
   static constexpr int NF = RKFields<JOB>::value;
   static constexpr int FIELDS = (S::FP < NF ? S::FP : NF) * S::REGION;
-  static constexpr int TW = N <= 2048 ? OctTab<N>::SIZE : 1;
+  static constexpr bool STEP_JOB =
+      JOB == RK_THETA || JOB == RK_THETA_CL || JOB == RK_SCALAR || JOB == RK_VEL;
+  static constexpr bool TWL =
+      N <= 2048 || (STEP_JOB && (FIELDS + OctTab<N>::SIZE) * (int)sizeof(cd) <= 160 * 1024);
+  static constexpr int TW = TWL ? OctTab<N>::SIZE : 1;
 };
 
 // One row-kernel block: x-tile bx of field slice by (see ck_body).
@@ -1135,7 +1145,10 @@ static hipError_t rk_by_job_v(int job, const RKArgs& a, int ny, hipStream_t s, c
 // index 0 is the default.  Radix-8 tiles 0-2 where a block of 128 threads
 // holds several rows (at N = 1024 the default column block is one row, so
 // there is no smaller tile), and from N = 64 up the radix-16 variant 3 and,
-// up to N = 1024, the radix-4 variant 4.
+// up to N = 1024, the radix-4 variant 4 (and up to N = 512 the field-group
+// variant 5: at N = 1024 its 512-lane blocks need more than 128 registers
+// per lane in the later stages, so only one fits per CU; measured 10.8 ms
+// per 32 steps against 7.9 ms for the radix-4 pair).
 template <int N>
 constexpr int n_variants() {
   return N <= 32 ? 3 : N <= 512 ? 6 : N == 1024 ? 4 : 2;

@@ -1,6 +1,7 @@
 """Large grids (N = 2048 .. 8192), where the kernels take their large-N
-paths: one field per lockstep group, twiddles from the global table,
-1024-lane column blocks at N = 8192.  The oracle is too slow beyond N = 2048,
+paths: one field per lockstep group, column-kernel twiddles from the
+global table (the row kernels keep an LDS table of 9-18 KB), 1024-lane
+column blocks at N = 8192.  The oracle is too slow beyond N = 2048,
 so larger sizes are checked through size-independent properties: transform
 round trips and the exact decay of a single |k| shell (its Jacobian vanishes,
 so only the linear operator acts).  The forced N = 1024 benchmark workload is
