@@ -586,11 +586,48 @@ __device__ __forceinline__ void fft_core16(cd (&v)[G][16], cd* buf, int gs, int 
   }
 }
 
+// 4 x 4 transpose across the four 16-lane rows of a wave: afterwards lane
+// (row r, l) holds in x[h] what lane (row h, l) held in x[r].  Two rounds of
+// the gfx950 row swaps, one 32-bit word at a time: v_permlane32_swap trades
+// rows 2-3 of its first operand for rows 0-1 of its second, and
+// v_permlane16_swap the odd rows of its first for the even rows of its
+// second.  Registers only: no LDS, no barrier, no arithmetic (the values
+// move bit for bit).  Every lane of the wave must be active.
+__device__ __forceinline__ unsigned cd_word(const cd& z, int c) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(c < 2 ? z.x : z.y);
+  return (c & 1) ? (unsigned)(u >> 32) : (unsigned)u;
+}
+__device__ __forceinline__ void xpose_rows4(cd (&x)[4]) {
+  unsigned w[4][4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const auto a = __builtin_amdgcn_permlane32_swap(cd_word(x[0], c), cd_word(x[2], c), false, false);
+    const auto b = __builtin_amdgcn_permlane32_swap(cd_word(x[1], c), cd_word(x[3], c), false, false);
+    const auto lo = __builtin_amdgcn_permlane16_swap(a[0], b[0], false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap(a[1], b[1], false, false);
+    w[0][c] = lo[0];
+    w[1][c] = lo[1];
+    w[2][c] = hi[0];
+    w[3][c] = hi[1];
+  }
+#pragma unroll
+  for (int h = 0; h < 4; ++h) {
+    x[h].x = __longlong_as_double((long long)(((unsigned long long)w[h][1] << 32) | w[h][0]));
+    x[h].y = __longlong_as_double((long long)(((unsigned long long)w[h][3] << 32) | w[h][2]));
+  }
+}
+
 // The same over L/4 lanes with 4 elements per lane (lane t holds elements
 // t + m*L/4, m = 0..3): radix-4 Stockham passes, then a radix-2 pass where
 // log2 L is odd (pairs q, q + 2).  Twice the lanes of the radix-8 layout for
 // the same transform: at grid sizes where one radix-8 wave per SIMD is all a
 // launch has, this gives each SIMD two waves to switch between.
+//
+// L = 256 (Q = 64): one transform per wave, lane t of the transform being
+// lane t of its wave (every caller lays lanes out as tid = b * Q + t).  The
+// exchange before the last pass (Ns = 16 -> 64) sends element r of lane
+// (row h, l) to element h of lane (row r, l): a transpose across the wave's
+// 16-lane rows, done in registers with xpose_rows4 instead of through LDS.
 template <int L, int DIR, int G, class TW>
 __device__ __forceinline__ void fft_core4(cd (&v)[G][4], cd* buf, int gs, int t, const TW& tws) {
   static_assert(L >= 4 && (L & (L - 1)) == 0, "power of two >= 4");
@@ -598,10 +635,11 @@ __device__ __forceinline__ void fft_core4(cd (&v)[G][4], cd* buf, int gs, int t,
   constexpr int LOG = __builtin_ctz(L);
   constexpr int P4 = LOG / 2;
   constexpr int REM = LOG % 2;
+  constexpr bool XROWS = (Q == 64) && (REM == 0);  // last exchange across rows
   int Ns = 1;
 #pragma unroll
   for (int p = 0; p < P4; ++p) {
-    if (p > 0) {
+    if (p > 0 && !(XROWS && p == P4 - 1)) {
 #pragma unroll
       for (int g = 0; g < G; ++g) {
 #pragma unroll
@@ -621,7 +659,10 @@ __device__ __forceinline__ void fft_core4(cd (&v)[G][4], cd* buf, int gs, int t,
 #pragma unroll
     for (int g = 0; g < G; ++g) dft4<DIR>(v[g][0], v[g][1], v[g][2], v[g][3]);
     const bool last = (p == P4 - 1) && (REM == 0);
-    if (!last) {
+    if (XROWS && p == P4 - 2) {
+#pragma unroll
+      for (int g = 0; g < G; ++g) xpose_rows4(v[g]);
+    } else if (!last) {
       __syncthreads();
       const int base = (t / Ns) * Ns * 4 + (t % Ns);
 #pragma unroll
