@@ -214,8 +214,10 @@ int gsqg_profile_advance(gsqg_plan* plan, double dt, int n_steps, int key_mode, 
  * transform different prologue fields; its row tile is the radix-4 one).
  * n_variants = 3 for n <= 32 (radix-8 tiles), 6 for 64 <= n <= 512 (the
  * three tiles, radix 16, radix 4, field groups), 4 for n = 1024
- * (default and taller radix-8 tile, radix 16, radix 4), 2 above (radix 8
- * and radix 16); gsqg_plan_shape_radix reports the radix of a shape index.  gsqg_autotune times
+ * (default and taller radix-8 tile, radix 16, radix 4), 3 above (radix 8,
+ * radix 16 and radix 4; at n = 8192 column shape 2 is the radix-8 tile, a
+ * radix-4 row would need 2048 lanes); gsqg_plan_shape_radix reports the
+ * radix of a shape index (the row kernel's where the two differ).  gsqg_autotune times
  * gsqg_advance over n_steps steps (graphs on, explicit key 0) for every
  * (column, row) shape pair, best of `repeats` runs each, keeps the fastest
  * pair in the plan and returns it in best[2]; ms_table (n_variants^2 host

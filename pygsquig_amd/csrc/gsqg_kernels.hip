@@ -38,7 +38,10 @@ namespace gsqg {
 // is listed in variant_id() below.
 template <int N, int V = 0>
 struct CKShape {
-  static constexpr int R = V == 3 ? 16 : (V == 4 || V == 5) ? 4 : 8;  // elements per lane
+  // elements per lane; variant 4 keeps radix 8 where radix 4 would need more
+  // than 1024 lanes per row (N = 8192), so that it names the row kernel's
+  // radix-4 tile alone there
+  static constexpr int R = V == 3 ? 16 : (V == 4 && N / 4 > 1024) ? 8 : (V == 4 || V == 5) ? 4 : 8;
   static constexpr int Q = N / R;                        // lanes per row
   static constexpr int R00 = (128 / Q) > 1 ? (128 / Q) : 1;
   // radix 4: one row per block down to one wave, then one wave per block
@@ -70,11 +73,12 @@ struct RKShape {
   static constexpr int W00 = (128 / Q) > 1 ? (128 / Q) : 1;
   // Radix-16 tiles at 256 lanes per x-line take two x-lines, so that each
   // row of a tile is 32 contiguous bytes of the ky-major fields; radix-4
-  // tiles take two x-lines, or one wave's worth where x-lines are shorter.
+  // tiles take two x-lines, or one wave's worth where x-lines are shorter,
+  // or one x-line where that alone has 512 lanes or more (N >= 4096).
   static constexpr int W0 = V == 1   ? 2 * W00
                             : V == 2 ? (W00 > 1 ? W00 / 2 : W00)
                             : V == 3 ? (Q >= 256 ? 2 : W00)
-                            : (V == 4 || V == 5) ? ((64 / Q) > 2 ? (64 / Q) : 2)
+                            : (V == 4 || V == 5) ? (Q >= 512 ? 1 : (64 / Q) > 2 ? (64 / Q) : 2)
                                      : W00;
   static constexpr int WX = W0 < N ? W0 : N;  // x-rows per block
   static constexpr int NT = WX * Q;
@@ -1144,18 +1148,19 @@ static hipError_t rk_by_job_v(int job, const RKArgs& a, int ny, hipStream_t s, c
 // Shape variants compiled for grid size N, in the autotuner's index order;
 // index 0 is the default.  Radix-8 tiles 0-2 where a block of 128 threads
 // holds several rows (at N = 1024 the default column block is one row, so
-// there is no smaller tile), and from N = 64 up the radix-16 variant 3 and,
-// up to N = 1024, the radix-4 variant 4 (and up to N = 512 the field-group
+// there is no smaller tile), and from N = 64 up the radix-16 variant 3 and
+// the radix-4 variant 4 (at N = 8192 a radix-4 row tile beside the radix-8
+// column tile), and up to N = 512 the field-group
 // variant 5: at N = 1024 its 512-lane blocks need more than 128 registers
 // per lane in the later stages, so only one fits per CU; measured 10.8 ms
 // per 32 steps against 7.9 ms for the radix-4 pair).
 template <int N>
 constexpr int n_variants() {
-  return N <= 32 ? 3 : N <= 512 ? 6 : N == 1024 ? 4 : 2;
+  return N <= 32 ? 3 : N <= 512 ? 6 : N == 1024 ? 4 : 3;
 }
 template <int N>
 constexpr int variant_id(int k) {
-  if (N >= 2048) return k == 0 ? 0 : 3;
+  if (N >= 2048) return k == 0 ? 0 : k == 1 ? 3 : 4;  // radix 8, 16, 4
   if (N == 1024) return k <= 1 ? k : k + 1;  // 0, 1, 3, 4
   return k;  // 0, 1, 2 (and 3, 4, 5 from N = 64)
 }
@@ -1193,7 +1198,7 @@ int ck_blocks_n(int var) {
 template <int N, int K = 0>
 int radix_n(int var) {
   if constexpr (K < n_variants<N>()) {
-    if (var == K || K + 1 == n_variants<N>()) return CKShape<N, variant_id<N>(K)>::R;
+    if (var == K || K + 1 == n_variants<N>()) return RKShape<N, variant_id<N>(K)>::R;
     return radix_n<N, K + 1>(var);
   }
   return 0;

@@ -14,9 +14,9 @@ from oracle import reference_sqg as R
 
 pytestmark = pytest.mark.gpu
 TWO_PI = 2 * np.pi
-# shape variants per grid size (gsqg.h: radix-8 tiles, plus radix 16 from N = 64,
-# radix 4 from N = 64 to 1024, and the field-group column tile from N = 64 to 512)
-N_VARIANTS = {16: 3, 32: 3, 64: 6, 128: 6, 256: 6, 512: 6, 1024: 4, 2048: 2, 4096: 2, 8192: 2}
+# shape variants per grid size (gsqg.h: radix-8 tiles, plus radix 16 and radix 4
+# from N = 64, and the field-group column tile from N = 64 to 512)
+N_VARIANTS = {16: 3, 32: 3, 64: 6, 128: 6, 256: 6, 512: 6, 1024: 4, 2048: 3, 4096: 3, 8192: 3}
 
 
 @pytest.fixture(scope="module")
@@ -156,13 +156,13 @@ def test_ensemble_autotune_and_validation(pg):
     tuned = ens.autotune(1e-3, n_steps=4, repeats=1)
     assert set(tuned) == {64, 4096}
     assert len(tuned[64][1]) == 36
-    assert len(tuned[4096][1]) == 4  # radix 8 and radix 16 above N = 1024
+    assert len(tuned[4096][1]) == 9  # radix 8, 16 and 4 above N = 1024
     got = ens.multistep(st, 5, 1e-3)
     for a, b in zip(got["theta_hat"], ref["theta_hat"]):
         rel = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
         assert rel < 1e-13, rel
     with pytest.raises(ValueError, match="shape variant out of range"):
-        ens.plans[4096].set_shape(2, 0)
+        ens.plans[4096].set_shape(3, 0)
 
 
 def test_solver_autotune_with_scalars(pg):

@@ -86,10 +86,11 @@ def forced2048_oracle():
     return theta0, s0, sto["theta_hat"], sc["a"]
 
 
-@pytest.mark.parametrize("shapes", [(0, 0), (1, 1), (0, 1), (1, 0)])
+@pytest.mark.parametrize("shapes", [(0, 0), (1, 1), (0, 1), (1, 0), (2, 2), (0, 2)])
 def test_forced_scalar_step_vs_oracle_2048(pg, forced2048_oracle, shapes):
     """Two forced, damped steps with a passive scalar at N = 2048 (IF), in
-    each pair of the two tile shapes there: radix 8 (0) and radix 16 (1)."""
+    pairs of the three tile shapes there: radix 8 (0), radix 16 (1) and
+    radix 4 (2)."""
     from pygsquig_amd.scalars import gSQGSolverWithScalars
 
     N, dt, n = _F2048["N"], _F2048["dt"], _F2048["n"]
@@ -165,3 +166,36 @@ def test_forced_1024_workload_vs_oracle(pg, forced1024_oracle, shapes):
     for k in blobs:
         gs = st["scalar_state"].scalars[k].cpu().numpy()
         assert np.linalg.norm(gs - sco[k]) / np.linalg.norm(sco[k]) < 1e-12, k
+
+
+def test_row_tiles_agree_8192(pg):
+    """N = 8192 with ring forcing, damping and two scalars: one step in each
+    row tile (radix 8, radix 16, and the 1024-lane radix-4 tile that holds
+    one x-line; column shape 2 is the radix-8 tile at this size) agrees with
+    the default shapes to rounding.  The default pair is checked against the
+    oracle at this size by test_gpu_config3."""
+    from pygsquig_amd.scalars import gSQGSolverWithScalars
+
+    N, alpha, dt = 8192, 0.5, 1e-4
+    nu_d = 1e-8 * (512 / N) ** 8
+    g = pg.make_grid(N, TWO_PI)
+    theta0 = torch.from_numpy(ics.kolmogorov_53(N, seed=2, alpha=alpha)).cuda()
+    blobs = {f"s{i}": torch.from_numpy(ics.gaussian_blob(N, c, TWO_PI / 16))
+             for i, c in enumerate(ics.SCALAR_CENTRES[:2])}
+    solver = gSQGSolverWithScalars(g, alpha, 0.0, 8, scheme="if",
+                                   passive_scalars={k: {"kappa": 0.0} for k in blobs})
+    f = pg.RingForcing(kf=100, dk=2, epsilon=0.1)
+    d = pg.CombinedDamping(mu=0.1, kf=100, nu_p=nu_d, p=8)
+    out = {}
+    for shapes in [(0, 0), (0, 2), (2, 2), (0, 1)]:
+        print(f"[row tiles 8192] shapes {shapes}", flush=True)
+        solver._plan(f, d).set_shape(*shapes)
+        st = solver.initialize(theta0, scalar_init=blobs)
+        st = solver.step(st, dt, forcing=f, damping=d, key=7)
+        out[shapes] = [st["theta_hat"].clone()] + [st["scalar_state"].scalars[k].clone()
+                                                   for k in blobs]
+        del st
+        assert solver.check_finite(f, d)
+    for shapes, fields in out.items():
+        for a, b in zip(fields, out[(0, 0)]):
+            assert rel(a, b) < 1e-13, (shapes, rel(a, b))
