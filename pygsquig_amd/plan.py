@@ -283,7 +283,9 @@ class Plan:
         return c.value, r.value, n.value
 
     def shape_radix(self, shape: int) -> int:
-        """Elements per lane (the FFT radix, 8 or 16) of shape index ``shape``."""
+        """Elements per lane (the FFT radix: 4, 8 or 16) of shape index
+        ``shape``; the row kernel's where the column tile differs (at N = 8192
+        shape 2 pairs a radix-4 row tile with the radix-8 column tile)."""
         r = ctypes.c_int(0)
         _lib.check(self.lib.gsqg_plan_shape_radix(self.handle, int(shape), ctypes.byref(r)),
                    "gsqg_plan_shape_radix")
