@@ -40,6 +40,33 @@ def test_fft2_ifft2(pg, N):
     assert rel(y, np.fft.ifft2(Y).real) < 1e-14
 
 
+@pytest.mark.parametrize("N", [256, 512])
+def test_fft2_ifft2_every_shape(pg, N):
+    """The API transforms in every tile shape of N = 256 and 512 against
+    numpy's c2c.  The radix-4 shapes there hold one transform per wave
+    (N = 256 columns, N = 512 half-length rows), whose last exchange is the
+    row-swap transpose instead of LDS."""
+    from pygsquig_amd.plan import Plan
+
+    g = pg.make_grid(N, TWO_PI)
+    plan = Plan(g, 1.0, 0.0, 8)
+    rs = np.random.default_rng(N + 1)
+    f = rs.standard_normal((N, N))
+    Y = rs.standard_normal((N, N)) + 1j * rs.standard_normal((N, N))
+    Xn, yn = np.fft.fft2(f), np.fft.ifft2(Y).real
+    nv = plan.shape()[2]
+    for c in range(nv):
+        for r in range(nv):
+            print(f"[fft2/ifft2 N={N} shapes ({c},{r})]", flush=True)
+            plan.set_shape(c, r)
+            X = torch.empty((N, N), dtype=torch.complex128, device="cuda")
+            plan.fft2(torch.from_numpy(f).cuda(), X)
+            y = torch.empty((N, N), dtype=torch.float64, device="cuda")
+            plan.ifft2(torch.from_numpy(Y).cuda(), y)
+            assert rel(X, Xn) < 1e-14, (c, r)
+            assert rel(y, yn) < 1e-14, (c, r)
+
+
 @pytest.mark.parametrize("N,alpha", [(32, 1.0), (64, 0.5), (128, 2.0), (256, 1.0), (64, 0.0)])
 def test_velocity(pg, N, alpha):
     print(f"[velocity N={N} alpha={alpha}]", flush=True)
