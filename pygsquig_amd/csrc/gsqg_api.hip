@@ -539,7 +539,7 @@ static int launch_scalar_rk(gsqg_plan* p, const double* u, const double* v, hipS
   // Large grids: one block per x-line tile transforms every scalar against
   // the velocity it loads once (grids of N >= 2048 have blocks to spare);
   // below, each scalar is a y-slice of the launch, for parallelism.
-  const bool loop = p->N >= 2048 && p->n_scalars > 1;
+  const bool loop = p->N >= GSQG_SCALAR_LOOP_N && p->n_scalars > 1;
   a.n_loop = loop ? p->n_scalars : 1;
   HIPCHK(rk(p, RK_SCALAR, a, loop ? 1 : p->n_scalars, st));
   return GSQG_OK;
