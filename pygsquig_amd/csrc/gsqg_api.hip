@@ -123,6 +123,8 @@ struct gsqg_plan {
   // branch (cap2) beside theta's stages 2-4
   hipStream_t cap = nullptr;
   hipStream_t cap2 = nullptr;
+  // eager steps run their scalar step on this stream beside theta's
+  hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipGraphExec_t gexec = nullptr;     // one captured step
   hipGraphExec_t gexec_k = nullptr;   // kGraphSteps captured steps
@@ -165,6 +167,7 @@ static void free_plan(gsqg_plan* p) {
   if (p->gexec_k) (void)hipGraphExecDestroy(p->gexec_k);
   if (p->cap) (void)hipStreamDestroy(p->cap);
   if (p->cap2) (void)hipStreamDestroy(p->cap2);
+  if (p->side) (void)hipStreamDestroy(p->side);
   if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
   if (p->ev_join) (void)hipEventDestroy(p->ev_join);
   delete p;
@@ -318,6 +321,7 @@ extern "C" int gsqg_plan_create(const gsqg_plan_desc* d, gsqg_plan** out) {
   if (p->n_scalars > 0) HIPCHK(hipMemset(p->s_th, 0, h * p->n_scalars * sizeof(double2)));
   HIPCHK(hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking));
   HIPCHK(hipStreamCreateWithFlags(&p->cap2, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
   HIPCHK(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming));
   HIPCHK(hipDeviceSynchronize());
@@ -625,6 +629,9 @@ static int capture_steps(gsqg_plan* p, int nsteps, double dt, bool derived, bool
   (void)hipGraphDestroy(g);
   if (ie != hipSuccess)
     return fail(GSQG_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+  // upload at capture rather than at the first launch
+  HIPCHK(hipGraphUpload(*out, p->cap));
+  HIPCHK(hipStreamSynchronize(p->cap));
   return GSQG_OK;
 }
 
@@ -682,6 +689,9 @@ extern "C" int gsqg_advance(gsqg_plan* p, double dt, int n_steps, int key_mode, 
     if ((rc = theta_row(p, rk_theta(p, 1, derived ? 1 : 0, false), p->n_mem, st, true))) return rc;
     if ((rc = theta_rest(p, dt, derived, !last || sc, last ? 0 : 1, st))) return rc;
   }
+  // Eager steps fork their scalar step onto the plan's side stream, as the
+  // captured steps do (not while timing every launch, which wants them serial).
+  hipStream_t eager_side = p->prof ? nullptr : p->side;
   // middle steps 1 .. n-2
   const int n_mid = n_steps >= 2 ? n_steps - 2 : 0;
   if (n_mid > 0) {
@@ -695,21 +705,23 @@ extern "C" int gsqg_advance(gsqg_plan* p, double dt, int n_steps, int key_mode, 
       for (int s = 0; s < n1; ++s) {
         if (graph1) {
           HIPCHK(hipGraphLaunch(p->gexec, st));
-        } else if ((rc = step_body(p, dt, derived, sc, true, 1, true, st, nullptr,
+        } else if ((rc = step_body(p, dt, derived, sc, true, 1, true, st, eager_side,
                                    cl_step_evals(p, s)))) {
           return rc;
         }
       }
     } else {
       for (int s = 0; s < n_mid; ++s) {
-        if ((rc = step_body(p, dt, derived, sc, true, 1, true, st, nullptr, cl_step_evals(p, s))))
+        if ((rc = step_body(p, dt, derived, sc, true, 1, true, st, eager_side,
+                            cl_step_evals(p, s))))
           return rc;
       }
     }
   }
   // last step (if n >= 2)
   if (n_steps >= 2) {
-    if ((rc = step_body(p, dt, derived, sc, sc, 0, true, st, nullptr, cl_step_evals(p, n_mid))))
+    if ((rc = step_body(p, dt, derived, sc, sc, 0, true, st, eager_side,
+                        cl_step_evals(p, n_mid))))
       return rc;
   }
   if (sc) {
