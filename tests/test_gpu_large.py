@@ -199,3 +199,70 @@ def test_row_tiles_agree_8192(pg):
     for shapes, fields in out.items():
         for a, b in zip(fields, out[(0, 0)]):
             assert rel(a, b) < 1e-13, (shapes, rel(a, b))
+
+
+def test_forced_1024_workload_48_steps_vs_oracle(pg):
+    """The benchmarked workload itself: N = 1024, alpha = 1, IF scheme, ring
+    forcing kf = 40 (seed 42, derived step keys), CombinedDamping (mu = 0.1,
+    nu_p = 1e-8 (512/N)^8) and two scalars, in the tile shapes the autotuner
+    picks there (3, 3), as graph-replayed multistep calls against the oracle
+    stepped one at a time (scipy.fft on the host's cores).
+
+    Through the first 16-step graph theta_hat agrees to 1e-13 (measured
+    1.5e-15).  From step 17 on, every pair of trajectories of this turbulent
+    state separates about tenfold per four steps: two GPU tile shapes, which
+    differ only in rounding, as much as GPU and oracle.  So at step 48 the test
+    bounds the GPU-oracle distance by the oracle's own divergence from a start
+    perturbed by one part in 1e15 (10x), instead of a fixed 1e-12."""
+    from oracle import cpu_baseline
+    from oracle import rng as orng
+    from pygsquig_amd.scalars import gSQGSolverWithScalars
+
+    N, dt, seed = 1024, 5e-4, 42
+    nu_d = 1e-8 * (512 / N) ** 8
+    theta0, blobs = _f1024_inputs()
+    g = pg.make_grid(N, TWO_PI)
+    solver = gSQGSolverWithScalars(g, 1.0, 0.0, 8, scheme="if",
+                                   passive_scalars={k: {"kappa": 0.0} for k in blobs})
+    f = pg.RingForcing(kf=40, dk=2, epsilon=0.1, seed=seed)
+    d = pg.CombinedDamping(mu=0.1, kf=40, nu_p=nu_d, p=8)
+    solver._plan(f, d).set_shape(3, 3)
+    st = solver.initialize(torch.from_numpy(theta0).cuda(),
+                           scalar_init={k: torch.from_numpy(b) for k, b in blobs.items()})
+    print("[1024 x 48] gpu", flush=True)
+    st = solver.multistep(st, 16, dt, forcing=f, damping=d)
+    got16 = st["theta_hat"].cpu().numpy()
+    st = solver.multistep(st, 32, dt, forcing=f, damping=d)
+    got48 = st["theta_hat"].cpu().numpy()
+    del st
+
+    def oracle_run(th0):
+        go = R.make_grid(N, TWO_PI)
+        so = R.Solver(go, 1.0, 0.0, 8, scheme="if")
+        fo = R.RingForcing(kf=40, dk=2, epsilon=0.1)
+        do = R.CombinedDamping(mu=0.1, kf=40, nu_p=nu_d, p=8)
+        sto = so.initialize(th0)
+        out = {}
+        for i in range(48):
+            k = orng.step_key(seed, i)
+            sto = so.step(sto, dt, forcing=lambda y, k=k: fo(y, key=k, grid=go), damping=do)
+            if i + 1 in (16, 48):
+                out[i + 1] = sto["theta_hat"]
+        return out
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    R.set_fft_engine(cpu_baseline.ScipyFFT(min(cpu_baseline.host_cores(), 16)))
+    try:
+        print("[1024 x 48] oracle", flush=True)
+        ref = oracle_run(theta0)
+        print("[1024 x 48] oracle, perturbed start", flush=True)
+        pert = oracle_run(theta0 * (1.0 + 1e-15))
+    finally:
+        R.set_fft_engine(None)
+    e16, e48, s48 = rel(got16, ref[16]), rel(got48, ref[48]), rel(pert[48], ref[48])
+    print(f"[1024 x 48] gpu vs oracle: step 16 {e16:.3e}, step 48 {e48:.3e}; "
+          f"oracle vs perturbed oracle at 48 {s48:.3e}", flush=True)
+    assert e16 < 1e-13
+    assert e48 < 10 * s48
