@@ -101,8 +101,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default tile shapes (forced configuration)")
     ap.add_argument("--shapes", default=None, metavar="COL,ROW",
-                    help="set these column/row tile shapes instead of autotuning (forced "
-                         "configuration; profiles reuse the shapes a timed run picked)")
+                    help="set these column/row tile shapes instead of autotuning (forced, "
+                         "plain and large configurations; profiles reuse the shapes a timed "
+                         "run picked)")
     ap.add_argument("--copy-only", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates PMC byte counters)")
     ap.add_argument("--config", default="forced",
@@ -337,7 +338,10 @@ def bench_single(args):
                     "dt=1e-4")
     warm = min(args.warmup, 5) if args.config == "large" else args.warmup
     tuned = None
-    if not args.no_autotune:  # shapes picked by timing every pair (untimed here)
+    if args.shapes:
+        plan0 = solver._plan(None, None) if args.config == "plain" else solver._plan(f, d)
+        plan0.set_shape(*(int(x) for x in args.shapes.split(",")))
+    elif not args.no_autotune:  # shapes picked by timing every pair (untimed here)
         if args.config == "plain":
             tuned = solver.autotune(dt)
         else:  # N=8192: radix 8 and radix 16, a few steps per pair
